@@ -1,0 +1,164 @@
+/*
+ * la_hip.h -- C ABI of the MI355X (gfx950) dense matmul + factor-and-solve
+ * library that stands under rust-la's `la::Matrix<f64>` hot path.
+ *
+ * Conventions (shared by every entry point):
+ *   - Matrices are dense, ROW-MAJOR `double` (as `Matrix<T>` stores them,
+ *     src/matrix/mod.rs:26-30), described by (rows, cols) and, in the
+ *     device-resident variants, a leading dimension `ld` >= cols.
+ *   - Every function returns an int32 status from `la_status`.  LA_ERR_SHAPE
+ *     is the reference's `assert!` panic; LA_SINGULAR, LA_NOT_SQUARE,
+ *     LA_NOT_SYMMETRIC and LA_NOT_SPD are the reference's `Option::None`.
+ *   - Host-pointer entry points (`*_rm`) are synchronous: inputs are read from
+ *     host memory, outputs are written to caller-allocated host memory before
+ *     the call returns, matching the reference's synchronous semantics.
+ *   - Device-pointer entry points (`*_dev`) take HBM pointers, enqueue on the
+ *     context's stream and return without synchronising unless stated.
+ *   - One `la_ctx` per host thread; a context is not safe to share between
+ *     threads without external locking.
+ *   - `piv` is the reference's permutation VECTOR (row i of P*A is row piv[i]
+ *     of A, src/decomp/lu.rs:95-101), never LAPACK's swap list.
+ */
+#ifndef LA_HIP_H
+#define LA_HIP_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum la_status {
+    LA_OK = 0,
+    LA_ERR_SHAPE = 1,       /* reference: assert! panic                       */
+    LA_SINGULAR = 2,        /* reference: LUDecomposition::solve -> None      */
+    LA_NOT_SQUARE = 3,      /* reference: CholeskyDecomposition::new -> None  */
+    LA_NOT_SYMMETRIC = 4,   /* reference: CholeskyDecomposition::new -> None  */
+    LA_NOT_SPD = 5,         /* reference: CholeskyDecomposition::new -> None  */
+    LA_ERR_HIP = 6,         /* a HIP runtime call failed                      */
+    LA_ERR_NOMEM = 7,       /* device allocation failed or workspace cap hit  */
+    LA_NOT_CONVERGED = 8,   /* iterative refinement did not reach tolerance   */
+    LA_ERR_ARG = 9,         /* null pointer, bad enum, bad leading dimension  */
+    LA_ERR_TIMEOUT = 10     /* a cross-workgroup wait exceeded its spin limit */
+};
+
+/* Factorisation precision for the mixed-precision solvers. */
+enum la_prec {
+    LA_PREC_FP64 = 0,       /* plain FP64 factor (no refinement needed)         */
+    LA_PREC_FP32 = 1,       /* FP32 factor, FP32 MFMA trailing update           */
+    LA_PREC_BF16 = 2,       /* BF16 operands with per-column 2^k scaling, FP32 acc */
+    LA_PREC_FP16 = 3,       /* FP16 operands with per-matrix/column 2^k scaling  */
+    LA_PREC_FP32X3 = 4      /* FP32 factor whose trailing update runs as three
+                               FP16 MFMA products (hi*hi + hi*lo + lo*hi)      */
+};
+
+typedef struct la_ctx la_ctx;
+
+/* ---------------------------------------------------------------- context */
+int         la_ctx_create(int device, la_ctx **out);
+void        la_ctx_destroy(la_ctx *ctx);
+int         la_ctx_synchronize(la_ctx *ctx);
+void       *la_ctx_stream(la_ctx *ctx);          /* the hipStream_t in use  */
+int         la_ctx_set_stream(la_ctx *ctx, void *hip_stream);
+const char *la_status_string(int status);
+const char *la_ctx_last_error(la_ctx *ctx);
+int         la_device_info(la_ctx *ctx, int *num_cus, int64_t *hbm_bytes, int *clock_khz);
+/* Cap (bytes) on the context's grow-only device workspace; 0 = no cap. */
+int         la_ctx_set_workspace_limit(la_ctx *ctx, int64_t bytes);
+
+/* ----------------------------------------------- drop-in host entry points */
+
+/* C[m x p] = A[m x k] * B[k2 x p].
+ * Replaces `impl Mul<&Matrix<T>> for &Matrix<T>` (src/matrix/mod.rs:957-980)
+ * and its by-value forwards (mod.rs:982-998) for T = f64.
+ * LA_ERR_SHAPE if k != k2 (mod.rs:961) or any extent is 0 (mod.rs:209). */
+int la_dgemm_rm(la_ctx *ctx, int64_t m, int64_t k, int64_t k2, int64_t p,
+                const double *A, const double *B, double *C);
+
+/* dst = A * B into a caller-provided dst of (dst_rows x dst_cols).
+ * Replaces `Matrix::mmul` (src/matrix/mmatrix.rs:82-98); LA_ERR_SHAPE on the
+ * three asserts of mmatrix.rs:83-85. */
+int la_dmmul_rm(la_ctx *ctx, int64_t m, int64_t k, int64_t k2, int64_t p,
+                int64_t dst_rows, int64_t dst_cols,
+                const double *A, const double *B, double *dst);
+
+/* Partial-pivot LU of an m x n matrix.  LU receives the packed factors
+ * (strict lower = L with unit diagonal implied, diagonal+upper = U),
+ * piv the permutation vector (length m), *pospivsign the sign flag.
+ * Replaces `LUDecomposition::new` (src/decomp/lu.rs:104-168).  Never fails
+ * for a well-formed input (lu.rs:104: no Option). */
+int la_dgetrf_rm(la_ctx *ctx, int64_t m, int64_t n, const double *A,
+                 double *LU, uint64_t *piv, int *pospivsign);
+
+/* X (m x nx) solving L*U*X = B(piv,:) from a packed LU.
+ * Replaces `LUDecomposition::solve` (src/decomp/lu.rs:237-278):
+ * LA_ERR_SHAPE if b_rows != m (lu.rs:240) or m < n (the reference indexes out
+ * of bounds in is_non_singular, lu.rs:174-182); LA_SINGULAR (-> None) if any
+ * U diagonal entry is exactly zero (lu.rs:241-243). */
+int la_dgetrs_rm(la_ctx *ctx, int64_t m, int64_t n, const double *LU,
+                 const uint64_t *piv, int64_t b_rows, int64_t nx,
+                 const double *B, double *X);
+
+/* getrf + getrs in one device round trip: `Matrix::solve`
+ * (src/matrix/mod.rs:1030-1032).  With B = identity it is `Matrix::inverse`
+ * (mod.rs:1034-1037). */
+int la_dgesv_rm(la_ctx *ctx, int64_t n_rows, int64_t n_cols, const double *A,
+                int64_t b_rows, int64_t nx, const double *B, double *X);
+
+/* Cholesky factor L (full n x n, strict upper triangle zeroed).
+ * Replaces `CholeskyDecomposition::new` (src/decomp/cholesky.rs:56-110):
+ * LA_NOT_SQUARE (cholesky.rs:57-59), LA_NOT_SYMMETRIC (exact A[k][j] !=
+ * A[j][k], cholesky.rs:89-91) and LA_NOT_SPD (d <= 0, cholesky.rs:97-100)
+ * all map to the reference's None. */
+int la_dpotrf_rm(la_ctx *ctx, int64_t rows, int64_t cols, const double *A, double *L);
+
+/* X (n x nx) solving L*L^T*X = B.  Replaces `CholeskyDecomposition::solve`
+ * (src/decomp/cholesky.rs:116-144); LA_ERR_SHAPE if b_rows != n (:118). */
+int la_dpotrs_rm(la_ctx *ctx, int64_t n, const double *L, int64_t b_rows,
+                 int64_t nx, const double *B, double *X);
+
+/* ------------------------------------------- device-resident entry points
+ * Same computations on HBM pointers with explicit leading dimensions; they
+ * enqueue on the context stream.  Used by the benchmarks and by callers
+ * that keep matrices resident. */
+
+int la_dgemm_dev(la_ctx *ctx, int64_t m, int64_t n, int64_t k,
+                 double alpha, const double *A, int64_t lda,
+                 const double *B, int64_t ldb,
+                 double beta, double *C, int64_t ldc);
+
+/* In-place LU of the m x n matrix at A (ld lda).  d_piv (device, length m)
+ * receives the permutation vector; d_info (device int[2]) receives
+ * {pospivsign, status}; status is LA_OK or LA_ERR_TIMEOUT. */
+int la_dgetrf_dev(la_ctx *ctx, int64_t m, int64_t n, double *A, int64_t lda,
+                  uint64_t *d_piv, int32_t *d_info);
+
+/* Solve with a device-resident packed LU; X (m x nx, ld ldx) must not
+ * alias B.  If d_status is non-NULL it receives LA_SINGULAR when U has an
+ * exact zero on its diagonal (the reference's None, lu.rs:241-243). */
+int la_dgetrs_dev(la_ctx *ctx, int64_t m, int64_t n, const double *LU, int64_t ldlu,
+                  const uint64_t *d_piv, int64_t nx, const double *B, int64_t ldb,
+                  double *X, int64_t ldx, int32_t *d_status);
+
+/* In-place Cholesky of the n x n matrix at A (lower factor, upper zeroed);
+ * d_status (device int) receives LA_OK / LA_NOT_SYMMETRIC / LA_NOT_SPD. */
+int la_dpotrf_dev(la_ctx *ctx, int64_t n, double *A, int64_t lda, int32_t *d_status);
+
+int la_dpotrs_dev(la_ctx *ctx, int64_t n, const double *L, int64_t ldl,
+                  int64_t nx, const double *B, int64_t ldb, double *X, int64_t ldx);
+
+/* ------------------------------------------------------- test generators
+ * Seeded counter-based inputs (SplitMix64 finaliser over a counter).
+ * Element i of stream s under seed q is mix(K + (i+1)*G) with
+ * K = mix(q + (s+1)*G), G = 0x9E3779B97F4A7C15.  Uniform values are
+ * 2*((bits>>11)*2^-53) - 1; normals are Box-Muller over index pairs. */
+int la_fill_uniform_dev(la_ctx *ctx, uint64_t seed, uint64_t stream,
+                        int64_t count, double *out);
+int la_fill_normal_dev(la_ctx *ctx, uint64_t seed, uint64_t stream,
+                       int64_t count, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LA_HIP_H */

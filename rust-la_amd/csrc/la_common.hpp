@@ -1,0 +1,127 @@
+// Shared definitions for the gfx950 kernels and the C ABI layer.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+#include <algorithm>
+#include "../../include/la_hip.h"
+
+namespace lahip {
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+typedef _Float16 h8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
+
+// ---- cross-workgroup exchange without cache-wide fences -------------------
+// Payload written for other workgroups of the same kernel goes out with
+// device-coherent (sc1) stores, then each writer waits for its stores to
+// complete (s_waitcnt vmcnt(0)) before the workgroup barrier that precedes
+// the flag store.  Readers poll the flag with sc1 loads and fetch the
+// payload with sc1 loads issued after the flag was seen.  This orders the
+// exchange without the agent-scope acquire/release fences, which on gfx950
+// invalidate / write back the whole L2 (buffer_inv sc1 / buffer_wbl2 sc1).
+template <typename T>
+__device__ inline void st_dev(T *p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ inline T ld_dev(const T *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// wait until every store issued by this thread has completed
+__device__ inline void stores_done() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// Context: one device, one stream, a grow-only workspace, small device scratch
+// for status words.
+struct Workspace {
+    void *ptr = nullptr;
+    size_t size = 0;
+};
+
+}  // namespace lahip
+
+struct la_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int num_cus = 0;
+    int64_t hbm_bytes = 0;
+    int clock_khz = 0;
+    int64_t ws_limit = 0;
+    lahip::Workspace ws[8];     // independent grow-only workspaces (WS_* ids)
+    int32_t *d_scratch = nullptr;   // 4 KiB of device status/counter words
+    int32_t *h_pinned = nullptr;    // 4 KiB pinned host mirror
+    std::string last_error;
+};
+
+namespace lahip {
+
+// Workspace slots.  A slot may be re-allocated (grown) by ws_get, which
+// invalidates earlier pointers from the SAME slot, so each slot has one user
+// per API call.
+enum { WS_HOST = 0, WS_PANEL = 1, WS_SYNC = 2, WS_PIV = 3, WS_LP = 4, WS_IR = 5,
+       WS_BATCH = 6, WS_GEN = 7 };
+
+// Device scratch words (ctx->d_scratch, 1024 int32):
+//   [0]  sticky device error (LA_ERR_TIMEOUT from bounded waits)
+//   [8]  Cholesky flag word      [16] singular flag      [24..31] status words
+enum { SCR_ERR = 0, SCR_CHOL = 8, SCR_SING = 16, SCR_STATUS = 24, SCR_RED = 64 };
+
+int set_error(la_ctx *ctx, int status, const char *what);
+int hip_fail(la_ctx *ctx, hipError_t e, const char *what);
+// Returns a device buffer of at least `bytes` from workspace slot `id`.
+int ws_get(la_ctx *ctx, int id, size_t bytes, void **out);
+
+#define LA_HIP_CHECK(ctx, expr)                                         \
+    do {                                                                \
+        hipError_t _e = (expr);                                         \
+        if (_e != hipSuccess) return ::lahip::hip_fail((ctx), _e, #expr); \
+    } while (0)
+
+static inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// ---- helpers implemented in util.hip --------------------------------------
+uint64_t spin_limit_ticks(la_ctx *ctx);
+int fill_i32(la_ctx *ctx, int32_t *p, int64_t n, int32_t v);
+int fill_u64(la_ctx *ctx, uint64_t *p, int64_t n, uint64_t v);
+int iota_u64(la_ctx *ctx, uint64_t *p, int64_t n);
+int copy2d(la_ctx *ctx, int64_t rows, int64_t cols, const double *src, int64_t lds,
+           double *dst, int64_t ldd);
+// status := LA_SINGULAR if any of the first n diagonal entries is exactly 0
+int check_diag_zero(la_ctx *ctx, int64_t n, const double *LU, int64_t ld, int32_t *status);
+
+// ---- kernel launchers implemented in the .hip translation units ----------
+// GEMM: C = alpha*A*op(B) + beta*C, row-major.  trans_b: B is given as an
+// N x K row-major matrix (B^T).  lower_only: skip tiles strictly above the
+// diagonal (symmetric rank-k updates).
+template <typename T>
+int gemm_rm(la_ctx *ctx, int64_t M, int64_t N, int64_t K, T alpha,
+            const T *A, int64_t lda, const T *B, int64_t ldb, T beta,
+            T *C, int64_t ldc, bool trans_b, bool lower_only);
+
+// Blocked right-looking LU with partial pivoting in precision T, in place.
+// d_ipiv (device, min(m,n)) receives LAPACK-style swaps (row k exchanged
+// with row d_ipiv[k], global 0-based); d_perm (device, m, may be null) the
+// permutation vector; d_status LA_OK or LA_ERR_TIMEOUT.
+template <typename T>
+int getrf_core(la_ctx *ctx, int64_t m, int64_t n, T *A, int64_t lda, int64_t *d_ipiv,
+               uint64_t *d_perm, int32_t *d_status);
+// d_info[0] = pospivsign (1 = even number of swaps)
+int pivsign(la_ctx *ctx, const int64_t *d_ipiv, int64_t kmax, int32_t *d_info);
+
+// Blocked triangular solves, FP64 arithmetic, factor TF:
+//   ('L', trans=false, unit=true)  L y = b  (packed LU)
+//   ('U', false, false)            U x = y  (packed LU)
+//   ('L', false, false)            L y = b  (Cholesky)
+//   ('L', true,  false)            L^T x = y (Cholesky)
+template <typename TF>
+int trsv_blocked(la_ctx *ctx, char uplo, bool trans, bool unit, int64_t n,
+                 const TF *Tm, int64_t ldt, int64_t nrhs, double *X, int64_t ldx);
+int gather_rows(la_ctx *ctx, int64_t m, int64_t nx, const uint64_t *d_piv, const double *B,
+                int64_t ldb, double *X, int64_t ldx);
+
+int potrf_blocked(la_ctx *ctx, int64_t n, double *A, int64_t lda, int32_t *d_status);
+
+}  // namespace lahip

@@ -1,0 +1,307 @@
+"""ctypes binding of liblahip.so (C ABI declared in include/la_hip.h).
+
+Host-pointer calls take and return numpy arrays and mirror the reference's
+drop-in surface (Mul, mmul, LUDecomposition::new/solve,
+CholeskyDecomposition::new/solve).  Device-pointer calls take torch CUDA
+tensors (torch is only the device-memory and stream plumbing here).
+
+There is no fallback: if the library is missing this module raises on
+import of the library, never computes on the CPU.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblahip.so")
+
+LA_OK = 0
+LA_ERR_SHAPE = 1
+LA_SINGULAR = 2
+LA_NOT_SQUARE = 3
+LA_NOT_SYMMETRIC = 4
+LA_NOT_SPD = 5
+LA_ERR_HIP = 6
+LA_ERR_NOMEM = 7
+LA_NOT_CONVERGED = 8
+LA_ERR_ARG = 9
+LA_ERR_TIMEOUT = 10
+
+PREC_FP64, PREC_FP32, PREC_BF16, PREC_FP16, PREC_FP32X3 = 0, 1, 2, 3, 4
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_i64 = ctypes.c_int64
+_vp = ctypes.c_void_p
+
+
+class LaError(RuntimeError):
+    """A non-OK status from the library (LA_ERR_SHAPE is the reference's panic)."""
+
+    def __init__(self, status, msg=""):
+        super().__init__(f"status {status}: {msg}")
+        self.status = status
+
+
+class IrReport(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("iterations", ctypes.c_int32),
+                ("prec", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("backward_error", ctypes.c_double),
+                ("initial_backward_error", ctypes.c_double),
+                ("anorm_inf", ctypes.c_double), ("factor_ms", ctypes.c_double),
+                ("refine_ms", ctypes.c_double), ("total_ms", ctypes.c_double)]
+
+
+_lib = None
+
+
+def load():
+    """Load liblahip.so; raises if it was not built (no CPU fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+    L = ctypes.CDLL(LIB_PATH)
+    L.la_status_string.restype = ctypes.c_char_p
+    L.la_ctx_last_error.restype = ctypes.c_char_p
+    L.la_ctx_last_error.argtypes = [_vp]
+    L.la_ctx_create.argtypes = [ctypes.c_int, ctypes.POINTER(_vp)]
+    L.la_ctx_destroy.argtypes = [_vp]
+    L.la_ctx_synchronize.argtypes = [_vp]
+    L.la_ctx_stream.argtypes = [_vp]
+    L.la_ctx_stream.restype = _vp
+    L.la_ctx_set_stream.argtypes = [_vp, _vp]
+    L.la_device_info.argtypes = [_vp, ctypes.POINTER(ctypes.c_int), _i64p,
+                                 ctypes.POINTER(ctypes.c_int)]
+    L.la_ctx_set_workspace_limit.argtypes = [_vp, _i64]
+    L.la_dgemm_rm.argtypes = [_vp, _i64, _i64, _i64, _i64, _dp, _dp, _dp]
+    L.la_dmmul_rm.argtypes = [_vp] + [_i64] * 6 + [_dp, _dp, _dp]
+    L.la_dgetrf_rm.argtypes = [_vp, _i64, _i64, _dp, _dp, _u64p, ctypes.POINTER(ctypes.c_int)]
+    L.la_dgetrs_rm.argtypes = [_vp, _i64, _i64, _dp, _u64p, _i64, _i64, _dp, _dp]
+    L.la_dgesv_rm.argtypes = [_vp, _i64, _i64, _dp, _i64, _i64, _dp, _dp]
+    L.la_dpotrf_rm.argtypes = [_vp, _i64, _i64, _dp, _dp]
+    L.la_dpotrs_rm.argtypes = [_vp, _i64, _dp, _i64, _i64, _dp, _dp]
+    L.la_dgemm_dev.argtypes = [_vp, _i64, _i64, _i64, ctypes.c_double, _vp, _i64, _vp, _i64,
+                               ctypes.c_double, _vp, _i64]
+    L.la_dgetrf_dev.argtypes = [_vp, _i64, _i64, _vp, _i64, _vp, _vp]
+    L.la_dgetrs_dev.argtypes = [_vp, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _vp]
+    L.la_dpotrf_dev.argtypes = [_vp, _i64, _vp, _i64, _vp]
+    L.la_dpotrs_dev.argtypes = [_vp, _i64, _vp, _i64, _i64, _vp, _i64, _vp, _i64]
+    L.la_fill_uniform_dev.argtypes = [_vp, ctypes.c_uint64, ctypes.c_uint64, _i64, _vp]
+    L.la_fill_normal_dev.argtypes = [_vp, ctypes.c_uint64, ctypes.c_uint64, _i64, _vp]
+    for name, args in _OPTIONAL.items():
+        if hasattr(L, name):
+            getattr(L, name).argtypes = args
+    _lib = L
+    return L
+
+
+_OPTIONAL = {
+    "la_dgesv_ir_dev": [_vp, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, ctypes.c_int,
+                        ctypes.c_double, ctypes.c_int, ctypes.POINTER(IrReport)],
+    "la_dgesv_ir_rm": [_vp, _i64, _i64, _dp, _dp, _dp, ctypes.c_int, ctypes.c_double,
+                       ctypes.c_int, ctypes.POINTER(IrReport)],
+    "la_gen_svd_matrix_dev": [_vp, _i64, ctypes.c_double, ctypes.c_int, ctypes.c_uint64,
+                              ctypes.c_uint64, ctypes.c_uint64, _vp, _i64, _i64,
+                              ctypes.c_uint64, _vp, _vp],
+    "la_backward_error_dev": [_vp, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64,
+                              ctypes.POINTER(ctypes.c_double)],
+    "la_batch_plan_create": [_vp, ctypes.c_int, _i64, _i32p, _i32p, ctypes.c_int,
+                             ctypes.POINTER(_vp)],
+    "la_batch_plan_destroy": [_vp],
+    "la_batch_plan_describe": [_vp],
+    "la_dpotrf_potrs_batched_dev": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "la_dgesv_ir_batched_dev": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_double,
+                                ctypes.c_int, _vp, _vp, _vp],
+    "la_autotune": [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p],
+    "la_autotune_load": [_vp, ctypes.c_char_p],
+}
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _c(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+
+
+class Context:
+    """One la_ctx (device, stream, workspaces)."""
+
+    def __init__(self, device=0):
+        self.lib = load()
+        h = _vp()
+        st = self.lib.la_ctx_create(device, ctypes.byref(h))
+        if st:
+            raise LaError(st, "la_ctx_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.la_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st, what):
+        if st != LA_OK:
+            msg = self.lib.la_ctx_last_error(self.h).decode(errors="replace")
+            raise LaError(st, f"{what}: {msg}")
+
+    def sync(self):
+        self._check(self.lib.la_ctx_synchronize(self.h), "synchronize")
+
+    def stream(self):
+        return self.lib.la_ctx_stream(self.h)
+
+    def device_info(self):
+        cus, hbm, clk = ctypes.c_int(), _i64(), ctypes.c_int()
+        self.lib.la_device_info(self.h, ctypes.byref(cus), ctypes.byref(hbm), ctypes.byref(clk))
+        return cus.value, hbm.value, clk.value
+
+    # ---- host (drop-in) entry points ------------------------------------
+    def mul(self, a, b):
+        a, b = _c(a), _c(b)
+        c = np.empty((a.shape[0], b.shape[1]))
+        self._check(self.lib.la_dgemm_rm(self.h, a.shape[0], a.shape[1], b.shape[0], b.shape[1],
+                                         _d(a), _d(b), _d(c)), "dgemm_rm")
+        return c
+
+    def mmul(self, a, b, dst):
+        a, b = _c(a), _c(b)
+        assert dst.flags.c_contiguous and dst.dtype == np.float64
+        self._check(self.lib.la_dmmul_rm(self.h, a.shape[0], a.shape[1], b.shape[0], b.shape[1],
+                                         dst.shape[0], dst.shape[1], _d(a), _d(b), _d(dst)),
+                    "dmmul_rm")
+        return dst
+
+    def lu_new(self, a):
+        a = _c(a)
+        m, n = a.shape
+        lu = np.empty_like(a)
+        piv = np.empty(m, dtype=np.uint64)
+        sign = ctypes.c_int()
+        self._check(self.lib.la_dgetrf_rm(self.h, m, n, _d(a), _d(lu), piv.ctypes.data_as(_u64p),
+                                          ctypes.byref(sign)), "dgetrf_rm")
+        return lu, piv, bool(sign.value)
+
+    def lu_solve(self, lu, piv, b):
+        """Returns None where the reference returns None (singular)."""
+        lu, b = _c(lu), _c(b)
+        if b.ndim == 1:
+            b = b.reshape(-1, 1)
+        piv = np.ascontiguousarray(piv, dtype=np.uint64)
+        x = np.empty((lu.shape[0], b.shape[1]))
+        st = self.lib.la_dgetrs_rm(self.h, lu.shape[0], lu.shape[1], _d(lu),
+                                   piv.ctypes.data_as(_u64p), b.shape[0], b.shape[1], _d(b), _d(x))
+        if st == LA_SINGULAR:
+            return None
+        self._check(st, "dgetrs_rm")
+        return x
+
+    def solve(self, a, b):
+        a, b = _c(a), _c(b)
+        if b.ndim == 1:
+            b = b.reshape(-1, 1)
+        x = np.empty((a.shape[0], b.shape[1]))
+        st = self.lib.la_dgesv_rm(self.h, a.shape[0], a.shape[1], _d(a), b.shape[0], b.shape[1],
+                                  _d(b), _d(x))
+        if st == LA_SINGULAR:
+            return None
+        self._check(st, "dgesv_rm")
+        return x
+
+    def chol_new(self, a):
+        """Returns (L or None, status)."""
+        a = _c(a)
+        rows, cols = a.shape
+        l = np.empty((rows, rows)) if rows == cols else np.empty((1, 1))
+        st = self.lib.la_dpotrf_rm(self.h, rows, cols, _d(a), _d(l))
+        if st in (LA_NOT_SQUARE, LA_NOT_SYMMETRIC, LA_NOT_SPD):
+            return None, st
+        self._check(st, "dpotrf_rm")
+        return l, st
+
+    def chol_solve(self, l, b):
+        l, b = _c(l), _c(b)
+        if b.ndim == 1:
+            b = b.reshape(-1, 1)
+        x = np.empty_like(b)
+        self._check(self.lib.la_dpotrs_rm(self.h, l.shape[0], _d(l), b.shape[0], b.shape[1],
+                                          _d(b), _d(x)), "dpotrs_rm")
+        return x
+
+    # ---- device entry points (torch tensors) -----------------------------
+    def gemm_dev(self, A, B, C, alpha=1.0, beta=0.0):
+        m, k = A.shape
+        n = B.shape[1]
+        self._check(self.lib.la_dgemm_dev(self.h, m, n, k, alpha, _ptr(A), A.stride(0), _ptr(B),
+                                          B.stride(0), beta, _ptr(C), C.stride(0)), "dgemm_dev")
+
+    def getrf_dev(self, A, piv, info):
+        m, n = A.shape
+        self._check(self.lib.la_dgetrf_dev(self.h, m, n, _ptr(A), A.stride(0), _ptr(piv),
+                                           _ptr(info)), "dgetrf_dev")
+
+    def getrs_dev(self, LU, piv, B, X, status=None):
+        m, n = LU.shape
+        self._check(self.lib.la_dgetrs_dev(self.h, m, n, _ptr(LU), LU.stride(0), _ptr(piv),
+                                           B.shape[1], _ptr(B), B.stride(0), _ptr(X),
+                                           X.stride(0), _ptr(status)), "dgetrs_dev")
+
+    def potrf_dev(self, A, status):
+        n = A.shape[0]
+        self._check(self.lib.la_dpotrf_dev(self.h, n, _ptr(A), A.stride(0), _ptr(status)),
+                    "dpotrf_dev")
+
+    def potrs_dev(self, L, B, X):
+        n = L.shape[0]
+        self._check(self.lib.la_dpotrs_dev(self.h, n, _ptr(L), L.stride(0), B.shape[1], _ptr(B),
+                                           B.stride(0), _ptr(X), X.stride(0)), "dpotrs_dev")
+
+    def fill_uniform_dev(self, seed, stream, out):
+        self._check(self.lib.la_fill_uniform_dev(self.h, seed, stream, out.numel(), _ptr(out)),
+                    "fill_uniform")
+
+    def fill_normal_dev(self, seed, stream, out):
+        self._check(self.lib.la_fill_normal_dev(self.h, seed, stream, out.numel(), _ptr(out)),
+                    "fill_normal")
+
+    def gesv_ir_dev(self, A, B, X, prec, tol, max_iter):
+        rep = IrReport()
+        n = A.shape[0]
+        st = self.lib.la_dgesv_ir_dev(self.h, n, B.shape[1], _ptr(A), A.stride(0), _ptr(B),
+                                      B.stride(0), _ptr(X), X.stride(0), prec, tol, max_iter,
+                                      ctypes.byref(rep))
+        if st not in (LA_OK, LA_NOT_CONVERGED):
+            self._check(st, "dgesv_ir_dev")
+        return rep
+
+    def gen_svd_matrix_dev(self, n, kappa, nrefl, seed, A, u_stream=100, v_stream=200,
+                           b=None, xstar=None, b_stream=300):
+        nrhs = b.shape[1] if b is not None else 0
+        self._check(self.lib.la_gen_svd_matrix_dev(self.h, n, kappa, nrefl, seed, u_stream,
+                                                   v_stream, _ptr(A), A.stride(0), nrhs,
+                                                   b_stream, _ptr(b), _ptr(xstar)),
+                    "gen_svd_matrix")
+
+    def backward_error_dev(self, A, X, B):
+        eta = ctypes.c_double()
+        self._check(self.lib.la_backward_error_dev(self.h, A.shape[0], B.shape[1], _ptr(A),
+                                                   A.stride(0), _ptr(X), X.stride(0), _ptr(B),
+                                                   B.stride(0), ctypes.byref(eta)),
+                    "backward_error")
+        return eta.value
