@@ -1,0 +1,199 @@
+#!/usr/bin/env python3
+"""Headline benchmark: FP64-accurate dense solve on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): n = 16384,
+A = U diag(sigma) V^T with U, V products of 64 Householder reflectors
+(seeded normal vectors), sigma geometrically spaced from 1 to 1/kappa,
+kappa = 1e6, one standard-normal right-hand side.  A "step" is one complete
+FP64-accurate solve with inputs resident in HBM:
+  FP32-IR : convert A to FP32, blocked LU on FP32 MFMA, FP64 iterative
+            refinement to normwise backward error <= 1e-15 * n
+  FP64    : plain blocked FP64 LU with partial pivoting + triangular solves
+value = FP64-equivalent flop count (2/3 n^3 - n^2/2 + 5n/6 + 2 n^2) divided
+by the FP32-IR step time (TFLOP/s).  The plain FP64 solve is timed beside
+it on the same A and b.
+
+Also reported: backward and forward errors, refinement steps, the dominant
+kernel's roofline (HIP-event timing of every launch of that kernel class
+during one extra, instrumented solve), and the CPU oracle (the reference's
+algorithm restated in C, one core) on a bounded sample.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "rust-la_amd"))
+
+PEAKS = {"fp64": 78.6, "fp32": 157.3, "fp16": 2500.0, "hbm_gbs": 8000.0}
+
+
+def lu_solve_flop(n, nrhs=1):
+    return (2.0 / 3.0) * n ** 3 - 0.5 * n ** 2 + (5.0 / 6.0) * n + 2.0 * n ** 2 * nrhs
+
+
+def cpu_baseline(n_sample):
+    """The CPU oracle (C restatement of lu.rs:104-278, -O2, no FMA, one
+    thread) timed on an n_sample x n_sample uniform system."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    import numpy as np
+    a = oracle.uniform(7, 0, (n_sample, n_sample))
+    b = oracle.uniform(7, 2, (n_sample, 1))
+    try:
+        os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
+    except Exception:
+        pass
+    t0 = time.perf_counter()
+    lu, piv, _ = oracle.lu_new(a)
+    oracle.lu_solve(lu, piv, b)
+    dt = time.perf_counter() - t0
+    return {"value": lu_solve_flop(n_sample) / dt / 1e12, "unit": "TFLOP/s", "cores": 1,
+            "kind": "port",
+            "sample": f"LU with partial pivoting + solve, n={n_sample}, 1 RHS, uniform[-1,1] "
+                      f"({dt:.1f} s on one core; C restatement of lu.rs:104-278)",
+            "seconds": dt}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=16384)
+    ap.add_argument("--kappa", type=float, default=1e6)
+    ap.add_argument("--cpu-n", type=int, default=2048)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for {args.gpus}); every GPU box here has "
+              "one GPU and the solver is single-device.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import la_hip
+    ctx = la_hip.Context(0)
+    ctx.use_torch_stream()
+    n, kappa = args.n, args.kappa
+    dev = "cuda"
+    A = torch.empty((n, n), dtype=torch.float64, device=dev)
+    b = torch.empty((n, 1), dtype=torch.float64, device=dev)
+    xs = torch.empty_like(b)
+    ctx.gen_svd_matrix_dev(n, kappa, 64, 2025, A, b=b, xstar=xs)
+    X = torch.empty_like(b)
+    tol = 1e-15 * n
+    torch.cuda.synchronize()
+
+    # ---------------- FP32-IR (the measured path)
+    def ir_step():
+        return ctx.gesv_ir_dev(A, b, X, la_hip.PREC_FP32, tol, 50)
+
+    for _ in range(args.warmup):
+        ir_step()
+    torch.cuda.synchronize()
+    times, reps = [], []
+    t_all0 = time.perf_counter()
+    for _ in range(args.steps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        rep = ir_step()
+        e1.record()
+        torch.cuda.synchronize()
+        times.append(e0.elapsed_time(e1))
+        reps.append(rep)
+    wall = time.perf_counter() - t_all0
+    ms_ir = sum(times) / len(times)
+    rep = reps[-1]
+    fwd_ir = ((X - xs).abs().max() / xs.abs().max()).item()
+    eta_ir = ctx.backward_error_dev(A, X, b)
+
+    # ---------------- plain FP64 LU + solve on the same system
+    W = torch.empty_like(A)
+    piv = torch.empty(n, dtype=torch.int64, device=dev)
+    info = torch.zeros(4, dtype=torch.int32, device=dev)
+    st = torch.zeros(1, dtype=torch.int32, device=dev)
+    X64 = torch.empty_like(b)
+    t64 = []
+    for it in range(args.warmup + args.steps):
+        W.copy_(A)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        ctx.getrf_dev(W, piv, info)
+        ctx.getrs_dev(W, piv, b, X64, st)
+        e1.record()
+        torch.cuda.synchronize()
+        if it >= args.warmup:
+            t64.append(e0.elapsed_time(e1))
+    ms_64 = sum(t64) / len(t64)
+    eta_64 = ctx.backward_error_dev(A, X64, b)
+    fwd_64 = ((X64 - xs).abs().max() / xs.abs().max()).item()
+
+    # ---------------- instrumented solve: per-kernel-class HIP-event timing
+    ctx.reset_kernel_stats()
+    ctx.set_profiling(True)
+    ir_step()
+    torch.cuda.synchronize()
+    ctx.set_profiling(False)
+    stats = ctx.kernel_stats()
+    dom = max(stats.items(), key=lambda kv: kv[1][1])
+    name, (calls, tot_ms, flop, nbytes) = dom
+    avg_ms = tot_ms / calls
+    if name.startswith("gemm"):
+        peak = {"gemm64": PEAKS["fp64"], "gemm32": PEAKS["fp32"], "gemm16": PEAKS["fp16"]}[name]
+        achieved = flop / calls / (avg_ms * 1e-3) / 1e12
+        roof = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s"}
+    else:
+        achieved = nbytes / calls / (avg_ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": achieved, "peak": PEAKS["hbm_gbs"], "unit": "GB/s"}
+    roof["frac"] = roof["achieved"] / roof["peak"]
+    roof["traffic"] = None
+    roof["kernel"] = name
+    roof["avg_launch_ms"] = avg_ms
+    roof["launches_per_step"] = calls
+
+    flop = lu_solve_flop(n)
+    out = {
+        "metric": "FP64-accurate solve TFLOP/s: FP32-IR vs plain FP64 LU, n=16384; batched solves/s",
+        "value": flop / (ms_ir * 1e-3) / 1e12,
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_ir,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32-factor+fp64-refine",
+        "data": "synthetic: seeded Householder-product A = U diag(sigma) V^T, normal b",
+        "config": {"workload": f"dense solve n={n}, kappa={kappa:g}, 1 RHS, FP32 LU + FP64 IR "
+                               f"to eta<=1e-15*n", "n": n, "kappa": kappa, "nrhs": 1},
+        "fp32_ir": {"ms": ms_ir, "tflops": flop / (ms_ir * 1e-3) / 1e12,
+                    "iterations": rep.iterations, "backward_error": eta_ir,
+                    "forward_error_vs_xstar": fwd_ir, "factor_ms": rep.factor_ms,
+                    "refine_ms": rep.refine_ms, "status": rep.status},
+        "fp64_lu": {"ms": ms_64, "tflops": flop / (ms_64 * 1e-3) / 1e12,
+                    "backward_error": eta_64, "forward_error_vs_xstar": fwd_64,
+                    "frac_of_fp64_peak": flop / (ms_64 * 1e-3) / 1e12 / PEAKS["fp64"]},
+        "speedup_fp32ir_vs_fp64": ms_64 / ms_ir,
+        "roofline": roof,
+        "kernel_breakdown_ms": {k: round(v[1], 3) for k, v in stats.items()},
+        "wall_s_timed": wall,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_n)
+        out["cpu_baseline"]["host_nproc"] = os.cpu_count()
+        try:
+            model = [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo")
+                     if l.startswith("model name")][0]
+            out["cpu_baseline"]["host_cpu"] = model
+        except Exception:
+            pass
+    ctx.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -67,6 +67,29 @@ int         la_device_info(la_ctx *ctx, int *num_cus, int64_t *hbm_bytes, int *c
 /* Cap (bytes) on the context's grow-only device workspace; 0 = no cap. */
 int         la_ctx_set_workspace_limit(la_ctx *ctx, int64_t bytes);
 
+/* ------------------------------------------------ kernel timing harness
+ * When enabled, every launch of an instrumented kernel class is bracketed
+ * by HIP events on the context stream; la_ctx_kernel_stats() resolves them
+ * (synchronising the stream) and returns launch count, summed GPU time and
+ * the ALGORITHMIC flop and byte counts of those launches (what the math
+ * requires, not what the hardware moved). */
+enum la_kernel_class {
+    LA_K_GEMM64 = 0,    /* FP64 MFMA GEMM (Mul, trailing updates)            */
+    LA_K_GEMM32 = 1,    /* FP32 MFMA GEMM (FP32 trailing updates)            */
+    LA_K_GEMM16 = 2,    /* FP16/BF16-operand MFMA GEMM                       */
+    LA_K_PANEL = 3,     /* LU panel factorisation (both panel kernels)       */
+    LA_K_TRSV = 4,      /* blocked triangular solves                         */
+    LA_K_RESID = 5,     /* FP64 residual r = b - A x                         */
+    LA_K_TRSM = 6,      /* unit-lower TRSM of the LU U12 block               */
+    LA_K_ROWMOVE = 7,   /* row interchanges                                  */
+    LA_K_BATCH = 8,     /* batched persistent kernels                        */
+    LA_K_COUNT = 9
+};
+int la_ctx_set_profiling(la_ctx *ctx, int enable);
+int la_ctx_reset_kernel_stats(la_ctx *ctx);
+int la_ctx_kernel_stats(la_ctx *ctx, int kclass, int64_t *calls, double *total_ms,
+                        double *flop, double *bytes);
+
 /* ----------------------------------------------- drop-in host entry points */
 
 /* C[m x p] = A[m x k] * B[k2 x p].
@@ -148,6 +171,35 @@ int la_dpotrf_dev(la_ctx *ctx, int64_t n, double *A, int64_t lda, int32_t *d_sta
 int la_dpotrs_dev(la_ctx *ctx, int64_t n, const double *L, int64_t ldl,
                   int64_t nx, const double *B, int64_t ldb, double *X, int64_t ldx);
 
+/* ------------------------------------------ mixed-precision LU with FP64 IR
+ * No reference counterpart: solves A x = b to FP64 accuracy by factoring a
+ * low-precision copy of A and refining with FP64 residuals r = b - A x,
+ * corrections and a normwise backward-error check
+ *   eta = ||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf)   (per RHS, max).
+ * A is never modified. */
+typedef struct la_ir_report {
+    int32_t status;          /* LA_OK, LA_NOT_CONVERGED, LA_SINGULAR ...      */
+    int32_t iterations;      /* refinement steps taken                        */
+    int32_t prec;            /* la_prec used for the factorisation            */
+    int32_t reserved;
+    double  backward_error;  /* final eta (max over RHS columns)              */
+    double  initial_backward_error; /* eta of the first low-precision solve   */
+    double  anorm_inf;       /* ||A||_inf                                     */
+    double  factor_ms;       /* GPU time of the low-precision factorisation   */
+    double  refine_ms;       /* GPU time of the refinement loop               */
+    double  total_ms;        /* convert + factor + refine                     */
+} la_ir_report;
+
+int la_dgesv_ir_dev(la_ctx *ctx, int64_t n, int64_t nrhs,
+                    const double *A, int64_t lda, const double *B, int64_t ldb,
+                    double *X, int64_t ldx, int prec, double tol, int max_iter,
+                    la_ir_report *report);
+
+/* Host-pointer form of the same solver. */
+int la_dgesv_ir_rm(la_ctx *ctx, int64_t n, int64_t nrhs, const double *A,
+                   const double *B, double *X, int prec, double tol, int max_iter,
+                   la_ir_report *report);
+
 /* ------------------------------------------------------- test generators
  * Seeded counter-based inputs (SplitMix64 finaliser over a counter).
  * Element i of stream s under seed q is mix(K + (i+1)*G) with
@@ -157,6 +209,23 @@ int la_fill_uniform_dev(la_ctx *ctx, uint64_t seed, uint64_t stream,
                         int64_t count, double *out);
 int la_fill_normal_dev(la_ctx *ctx, uint64_t seed, uint64_t stream,
                        int64_t count, double *out);
+
+/* A = U diag(sigma) V^T (n x n, row-major, FP64) with U, V products of
+ * `nrefl` Householder reflectors from normal vectors (streams u_stream+i and
+ * v_stream+i) and sigma_i = kappa^(-i/(n-1)).  If xstar/b are non-NULL,
+ * b (n x nrhs) is filled from stream b_stream with normals and xstar gets
+ * the exact solution V diag(1/sigma) U^T b, evaluated through the
+ * reflectors. */
+int la_gen_svd_matrix_dev(la_ctx *ctx, int64_t n, double kappa, int nrefl,
+                          uint64_t seed, uint64_t u_stream, uint64_t v_stream,
+                          double *A, int64_t lda,
+                          int64_t nrhs, uint64_t b_stream, double *b, double *xstar);
+
+/* ------------------------------------------------------------ utilities */
+/* eta = ||B - A X||_inf / (||A||_inf ||X||_inf + ||B||_inf), FP64 on device. */
+int la_backward_error_dev(la_ctx *ctx, int64_t n, int64_t nrhs,
+                          const double *A, int64_t lda, const double *X, int64_t ldx,
+                          const double *B, int64_t ldb, double *eta);
 
 #ifdef __cplusplus
 }

@@ -50,9 +50,81 @@ int ws_get(la_ctx *ctx, int id, size_t bytes, void **out) {
     return LA_OK;
 }
 
+static hipEvent_t ev_take(la_ctx *ctx) {
+    if (!ctx->ev_pool.empty()) {
+        hipEvent_t e = ctx->ev_pool.back();
+        ctx->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+ProfScope::ProfScope(la_ctx *c, int kclass, double flop, double bytes) : ctx(c), on(false) {
+    if (!ctx || !ctx->prof) return;
+    rec.kclass = kclass;
+    rec.flop = flop;
+    rec.bytes = bytes;
+    rec.start = ev_take(ctx);
+    rec.stop = ev_take(ctx);
+    if (!rec.start || !rec.stop) return;
+    on = hipEventRecord(rec.start, ctx->stream) == hipSuccess;
+}
+
+ProfScope::~ProfScope() {
+    if (!on) return;
+    if (hipEventRecord(rec.stop, ctx->stream) == hipSuccess) ctx->prof_pending.push_back(rec);
+}
+
+static int prof_resolve(la_ctx *ctx) {
+    if (ctx->prof_pending.empty()) return LA_OK;
+    LA_HIP_CHECK(ctx, hipEventSynchronize(ctx->prof_pending.back().stop));
+    for (auto &r : ctx->prof_pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, r.start, r.stop) == hipSuccess) {
+            ProfStats &s = ctx->prof_stats[r.kclass];
+            s.calls += 1;
+            s.ms += ms;
+            s.flop += r.flop;
+            s.bytes += r.bytes;
+        }
+        ctx->ev_pool.push_back(r.start);
+        ctx->ev_pool.push_back(r.stop);
+    }
+    ctx->prof_pending.clear();
+    return LA_OK;
+}
+
 }  // namespace lahip
 
 using namespace lahip;
+
+extern "C" int la_ctx_set_profiling(la_ctx *ctx, int enable) {
+    if (!ctx) return LA_ERR_ARG;
+    int st = prof_resolve(ctx);
+    ctx->prof = enable != 0;
+    return st;
+}
+
+extern "C" int la_ctx_reset_kernel_stats(la_ctx *ctx) {
+    if (!ctx) return LA_ERR_ARG;
+    int st = prof_resolve(ctx);
+    for (auto &s : ctx->prof_stats) s = ProfStats();
+    return st;
+}
+
+extern "C" int la_ctx_kernel_stats(la_ctx *ctx, int kclass, int64_t *calls, double *total_ms,
+                                   double *flop, double *bytes) {
+    if (!ctx || kclass < 0 || kclass >= LA_K_COUNT) return LA_ERR_ARG;
+    int st = prof_resolve(ctx);
+    const ProfStats &s = ctx->prof_stats[kclass];
+    if (calls) *calls = s.calls;
+    if (total_ms) *total_ms = s.ms;
+    if (flop) *flop = s.flop;
+    if (bytes) *bytes = s.bytes;
+    return st;
+}
 
 // ------------------------------------------------------------------ context
 extern "C" int la_ctx_create(int device, la_ctx **out) {
@@ -98,6 +170,11 @@ extern "C" void la_ctx_destroy(la_ctx *ctx) {
     if (!ctx) return;
     hipSetDevice(ctx->device);
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    for (auto &r : ctx->prof_pending) {
+        hipEventDestroy(r.start);
+        hipEventDestroy(r.stop);
+    }
+    for (auto e : ctx->ev_pool) hipEventDestroy(e);
     for (auto &w : ctx->ws)
         if (w.ptr) hipFree(w.ptr);
     if (ctx->d_scratch) hipFree(ctx->d_scratch);

@@ -208,6 +208,10 @@ int gemm_rm(la_ctx *ctx, int64_t M, int64_t N, int64_t K, T alpha,
     const int64_t ntiles = (int64_t)tiles_m * tiles_n;
     if (ntiles > INT32_MAX) return set_error(ctx, LA_ERR_ARG, "gemm: too many tiles");
     dim3 grid((unsigned)ntiles), block(256);
+    const double flop = (lower_only ? 1.0 : 2.0) * (double)M * (double)N * (double)K;
+    const double bytes = (double)sizeof(T) * ((double)M * K + (double)K * N +
+                                              (beta != T(0) ? 2.0 : 1.0) * (double)M * N);
+    ProfScope ps(ctx, sizeof(T) == 8 ? LA_K_GEMM64 : LA_K_GEMM32, flop, bytes);
     if (trans_b)
         hipLaunchKernelGGL((k_gemm_rm<T, true>), grid, block, 0, ctx->stream, (int)M, (int)N,
                            (int)K, alpha, A, lda, B, ldb, beta, C, ldc, lower_only ? 1 : 0,

@@ -141,46 +141,67 @@ __device__ void panel_net_moves(int NB, int kglob, const int *s_piv, int64_t *sw
 
 // ------------------------------------------------------------- panel kernel
 // Factors the M x NB panel whose top-left element is A (row-major, lda),
-// global column/row offset kglob.  Workgroup w owns panel rows
-// [w*R, min(w*R+R, M)) in LDS for the whole kernel.
+// global column/row offset kglob, with P cooperative workgroups; workgroup
+// w owns panel rows [w*R, min(w*R+R, M)) in LDS for the whole kernel and
+// its first wavefront owns the per-column critical path (R <= 64*RPL).
 //
-// Per column j, every workgroup publishes its best candidate (key, row) in
-// slot[buf][w] (two 64-bit words, each carrying the step tag so that a torn
-// read is detected and retried) together with the candidate's full panel
-// row; the owner of row j also publishes row j.  After all P slots show the
-// current step, each workgroup reduces them to the same winner, swaps rows j
-// and p in its own rows, scales the multipliers and applies the rank-1
-// update.  Slots and row buffers are double-buffered by step parity: a
-// workgroup can only reach step s+2 after every workgroup published step
-// s+1, i.e. after every workgroup finished reading step s.
+// Exchange: per column j every workgroup publishes its best candidate
+// (ordered |value| key, row index, sign) and that row's NB values, and the
+// owner of row j publishes row j.  Every published 64-bit word carries the
+// step number (the key word a step-parity bit), so a reader validates each
+// word on its own and never needs the writer to order its stores: no
+// store-completion wait and no separate flag.  Doubles travel as two words
+// (low and high halves), floats as one.  Buffers are double-buffered by
+// step parity: a workgroup reaches its step-(s+2) publication only after
+// seeing every workgroup's step-(s+1) slot, which each publishes after it
+// finished reading step s.
 //
-// swapmap receives the net row movement of the panel: count, then pairs
-// (dst, src) of global rows, where dst's final content is src's content
-// before the panel.
-template <typename T>
+// Per column, once the winner p is known: rows j and p are swapped, then the
+// first wavefront forms the multipliers (true division, skipped for a zero
+// pivot), updates column j+1, picks the next local candidate, brings that
+// row (and row j+1 if owned) fully up to date and publishes step j+1 before
+// the whole workgroup applies the rest of the rank-1 update, so the bulk of
+// the arithmetic overlaps the exchange latency.
+template <typename T> struct PanelWords;
+template <> struct PanelWords<double> { static constexpr int W = 2; };
+template <> struct PanelWords<float> { static constexpr int W = 1; };
+constexpr int CAND_WORDS = 2 + 2 * NB_IN;        // slot words + row words
+
+__device__ inline void put_value(uint64_t *w, uint64_t tag32, double v) {
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    st_dev(w + 0, tag32 | (b & 0xffffffffull));
+    st_dev(w + 1, tag32 | (b >> 32));
+}
+__device__ inline void put_value(uint64_t *w, uint64_t tag32, float v) {
+    st_dev(w, tag32 | (uint64_t)__float_as_uint(v));
+}
+
+template <typename T, int RPL>
 __global__ __launch_bounds__(256)
-void k_getrf_panel(T *A, int64_t lda, int M, int NB, int R, int kglob,
-                   int64_t *ipiv, uint64_t *slots, T *rowbuf, T *diagbuf,
-                   int64_t *swapmap, int32_t *status, uint64_t spin_ticks)
+void k_getrf_panel(T *A, int64_t lda, int M, int NB, int R, int kglob, int64_t *ipiv,
+                   uint64_t *cand, uint64_t *diagw, int64_t *swapmap, int32_t *status,
+                   uint64_t spin_ticks)
 {
+    constexpr int W = PanelWords<T>::W;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int LDB = NB + 1;
     T *blk = reinterpret_cast<T *>(smem_raw);
-    T *urow = blk + (size_t)R * LDB;
-    T *drow = urow + NB;
+    __shared__ __align__(16) uint32_t s_urow[W * NB_IN];   // winner row (raw halves)
+    __shared__ __align__(16) uint32_t s_drow[W * NB_IN];   // old row j (raw halves)
     __shared__ uint64_t s_key[4];
     __shared__ int s_idx[4];
-    __shared__ int s_p;
     __shared__ int s_abort;
     __shared__ int s_piv[NB_IN];
+    __shared__ int s_skip[2];
+    const T *urow = reinterpret_cast<const T *>(s_urow);
+    const T *drow = reinterpret_cast<const T *>(s_drow);
 
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int w = blockIdx.x;
     const int P = gridDim.x;
     const int rowbase = w * R;
     const int nrows = min(R, M - rowbase);
 
-    // load owned rows
     for (int e = tid; e < nrows * NB; e += blockDim.x) {
         int r = e / NB, c = e - r * NB;
         blk[r * LDB + c] = A[(int64_t)(rowbase + r) * lda + c];
@@ -188,55 +209,69 @@ void k_getrf_panel(T *A, int64_t lda, int M, int NB, int R, int kglob,
     if (tid == 0) s_abort = 0;
     __syncthreads();
 
+    // publish the candidate of step s (wave 0 only): rows >= s of column s
+    auto publish = [&](int s, uint64_t key, int idx, bool neg) {
+        const uint32_t gs = (uint32_t)(kglob + s);
+        const int buf = gs & 1;
+        const uint64_t tag32 = (uint64_t)gs << 32;
+        uint64_t *slot = cand + ((size_t)buf * PANEL_MAXP + w) * CAND_WORDS;
+        if (idx != IDX_NONE) {
+            const int rl = idx - rowbase;
+            for (int c = lane; c < NB; c += 64) put_value(slot + 2 + W * c, tag32, blk[rl * LDB + c]);
+        }
+        if (s >= rowbase && s < rowbase + nrows) {
+            const int rl = s - rowbase;
+            uint64_t *dw = diagw + (size_t)buf * W * NB_IN;
+            for (int c = lane; c < NB; c += 64) put_value(dw + W * c, tag32, blk[rl * LDB + c]);
+        }
+        if (lane == 0) {
+            const uint64_t tagb = ((gs >> 1) & 1) ? TAG_BIT : 0;
+            st_dev(slot + 0, tagb | key);
+            st_dev(slot + 1, tag32 | (neg ? 0x80000000ull : 0ull) | (uint32_t)idx);
+        }
+    };
+    // local candidate of column s over rows >= s (wave 0, RPL rows per lane)
+    auto local_cand = [&](int s, uint64_t &key, int &idx, bool &neg) {
+        key = 0; idx = IDX_NONE;
+#pragma unroll
+        for (int q = 0; q < RPL; ++q) {
+            const int r = lane + 64 * q;
+            const int gr = rowbase + r;
+            if (r < nrows && gr >= s) {
+                const T v = blk[r * LDB + s];
+                uint64_t k = abs_key(v);
+                if (gr == s && v != v) k = KEY_MAX;
+                if (better(k, gr, key, idx)) { key = k; idx = gr; }
+            }
+        }
+        wave_argmax(key, idx);
+        neg = false;
+        if (idx != IDX_NONE) neg = blk[(idx - rowbase) * LDB + s] < T(0);
+    };
+
+    if (wave == 0) {
+        uint64_t key; int idx; bool neg;
+        local_cand(0, key, idx, neg);
+        publish(0, key, idx, neg);
+    }
+
     for (int j = 0; j < NB; ++j) {
         const uint32_t gs = (uint32_t)(kglob + j);
         const int buf = gs & 1;
-        const uint64_t tag = ((gs >> 1) & 1) ? TAG_BIT : 0;
+        const uint64_t tagb = ((gs >> 1) & 1) ? TAG_BIT : 0;
 
-        // 1. local candidate
-        uint64_t key = 0;
-        int idx = IDX_NONE;
-        for (int r = tid; r < nrows; r += blockDim.x) {
-            const int gr = rowbase + r;
-            if (gr < j) continue;
-            T v = blk[r * LDB + j];
-            uint64_t k = abs_key(v);
-            if (gr == j && v != v) k = KEY_MAX;
-            if (better(k, gr, key, idx)) { key = k; idx = gr; }
-        }
-        wg_argmax(key, idx, s_key, s_idx);
-
-        // 2. publish candidate row (and row j if owned)
-        T *myrow = rowbuf + ((size_t)buf * PANEL_MAXP + w) * NB;
-        if (idx != IDX_NONE) {
-            const int rl = idx - rowbase;
-            for (int c = tid; c < NB; c += blockDim.x) st_dev(myrow + c, blk[rl * LDB + c]);
-        }
-        if (j >= rowbase && j < rowbase + nrows) {
-            const int rl = j - rowbase;
-            T *dr = diagbuf + (size_t)buf * NB;
-            for (int c = tid; c < NB; c += blockDim.x) st_dev(dr + c, blk[rl * LDB + c]);
-        }
-        stores_done();
-        __syncthreads();
-        if (tid == 0) {
-            uint64_t *slot = slots + ((size_t)buf * PANEL_MAXP + w) * 2;
-            st_relaxed(slot + 0, tag | key);
-            st_relaxed(slot + 1, ((uint64_t)gs << 32) | (uint32_t)idx);
-        }
-
-        // 3. wait for all P slots of this step and reduce them
+        // 1. gather all P candidates of step j
         uint64_t ck = 0;
         int ci = IDX_NONE;
         if (tid < P) {
-            const uint64_t *slot = slots + ((size_t)buf * PANEL_MAXP + tid) * 2;
-            uint64_t t0 = wall_clock64();
+            const uint64_t *slot = cand + ((size_t)buf * PANEL_MAXP + tid) * CAND_WORDS;
+            const uint64_t t0 = wall_clock64();
             while (true) {
-                uint64_t w1 = ld_relaxed(slot + 1);
-                uint64_t w0 = ld_relaxed(slot + 0);
-                if ((uint32_t)(w1 >> 32) == gs && (w0 & TAG_BIT) == tag) {
+                const uint64_t w1 = ld_dev(slot + 1);
+                const uint64_t w0 = ld_dev(slot + 0);
+                if ((uint32_t)(w1 >> 32) == gs && (w0 & TAG_BIT) == tagb) {
                     ck = w0 & ~TAG_BIT;
-                    ci = (int)(uint32_t)w1;
+                    ci = (int)((uint32_t)w1 & 0x7fffffffu);
                     break;
                 }
                 if (ld_status(status) != 0) { s_abort = 1; break; }
@@ -251,8 +286,6 @@ void k_getrf_panel(T *A, int64_t lda, int M, int NB, int R, int kglob,
         __syncthreads();
         if (s_abort) return;
         wg_argmax(ck, ci, s_key, s_idx);
-        // winning panel row (relative); a well-formed exchange always yields
-        // j <= ci < M, anything else is reported and replaced by j
         int p = ci;
         if (p < j || p >= M) {
             if (tid == 0) atomicCAS((int *)status, 0, (int)LA_ERR_TIMEOUT);
@@ -260,60 +293,99 @@ void k_getrf_panel(T *A, int64_t lda, int M, int NB, int R, int kglob,
         }
         const int wp = p / R;
 
-        // 4. fetch the pivot row (and old row j for the owner of p)
-        const T *wrow = rowbuf + ((size_t)buf * PANEL_MAXP + wp) * NB;
-        for (int c = tid; c < NB; c += blockDim.x) urow[c] = ld_dev(wrow + c);
-        if (p != j && p >= rowbase && p < rowbase + nrows) {
-            const T *dr = diagbuf + (size_t)buf * NB;
-            for (int c = tid; c < NB; c += blockDim.x) drow[c] = ld_dev(dr + c);
+        // 2. fetch the winner row (and old row j for the owner of p)
+        const bool need_d = (p != j) && (p >= rowbase && p < rowbase + nrows);
+        if (tid < W * NB || (need_d && tid >= 128 && tid - 128 < W * NB)) {
+            const bool isd = tid >= 128;
+            const int widx = isd ? tid - 128 : tid;
+            const uint64_t *src = isd ? diagw + (size_t)buf * W * NB_IN + widx
+                                      : cand + ((size_t)buf * PANEL_MAXP + wp) * CAND_WORDS + 2 + widx;
+            const uint64_t t0 = wall_clock64();
+            uint64_t v;
+            while (true) {
+                v = ld_dev(src);
+                if ((uint32_t)(v >> 32) == gs) break;
+                if (ld_status(status) != 0) break;
+                if (wall_clock64() - t0 > spin_ticks) {
+                    atomicCAS((int *)status, 0, (int)LA_ERR_TIMEOUT);
+                    break;
+                }
+            }
+            (isd ? s_drow : s_urow)[widx] = (uint32_t)v;
         }
         if (tid == 0) {
             s_piv[j] = p;
             if (w == 0) ipiv[kglob + j] = (int64_t)kglob + p;
         }
         __syncthreads();
+        // 3. swap rows j and p inside the panel
         if (p != j) {
             if (j >= rowbase && j < rowbase + nrows) {
                 const int rl = j - rowbase;
                 for (int c = tid; c < NB; c += blockDim.x) blk[rl * LDB + c] = urow[c];
             }
-            if (p >= rowbase && p < rowbase + nrows) {
+            if (need_d) {
                 const int rl = p - rowbase;
                 for (int c = tid; c < NB; c += blockDim.x) blk[rl * LDB + c] = drow[c];
             }
         }
         __syncthreads();
 
-        // 5. multipliers (true division, skipped for a zero pivot) + rank-1
+        // 4. critical path (wave 0): multipliers, column j+1, next candidate
         const T piv = urow[j];
-        const int r_first = max(0, j + 1 - rowbase);
-        if (piv != T(0)) {
-            for (int r = r_first + tid; r < nrows; r += blockDim.x)
-                blk[r * LDB + j] = blk[r * LDB + j] / piv;
+        const int j1 = j + 1;
+        if (wave == 0) {
+#pragma unroll
+            for (int q = 0; q < RPL; ++q) {
+                const int r = lane + 64 * q;
+                if (r < nrows && rowbase + r > j) {
+                    T l = blk[r * LDB + j];
+                    if (piv != T(0)) l = l / piv;
+                    blk[r * LDB + j] = l;
+                    if (j1 < NB) blk[r * LDB + j1] -= l * urow[j1];
+                }
+            }
+            // the next reads cross lanes: keep the compiler from moving them
+            // above the writes (LDS itself executes one wave's ops in order)
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            if (j1 < NB) {
+                uint64_t key; int idx; bool neg;
+                local_cand(j1, key, idx, neg);
+                // bring the candidate row and row j1 fully up to date
+                const int rc = (idx != IDX_NONE) ? idx - rowbase : -1;
+                const int rd = (j1 >= rowbase && j1 < rowbase + nrows && j1 != idx) ? j1 - rowbase : -1;
+                for (int c = j1 + 1 + lane; c < NB; c += 64) {
+                    if (rc >= 0) blk[rc * LDB + c] -= blk[rc * LDB + j] * urow[c];
+                    if (rd >= 0) blk[rd * LDB + c] -= blk[rd * LDB + j] * urow[c];
+                }
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                publish(j1, key, idx, neg);
+                if (lane == 0) { s_skip[0] = rc; s_skip[1] = rd; }
+            }
         }
         __syncthreads();
-        const int ncols = NB - j - 1;
-        if (ncols > 0 && r_first < nrows) {
-            const int nr = nrows - r_first;
-            for (int e = tid; e < nr * ncols; e += blockDim.x) {
-                const int r = r_first + e / ncols;
-                const int c = j + 1 + e % ncols;
+        // 5. rest of the rank-1 update, overlapping the step-(j+1) exchange
+        if (j1 + 1 < NB) {
+            const int sk0 = s_skip[0], sk1 = s_skip[1];
+            const int r_first = max(0, j + 1 - rowbase);
+            const int ncol = NB - j1 - 1;
+            for (int e = tid; e < (nrows - r_first) * ncol; e += blockDim.x) {
+                const int r = r_first + e / ncol;
+                const int c = j1 + 1 + e % ncol;
+                if (r == sk0 || r == sk1) continue;
                 blk[r * LDB + c] -= blk[r * LDB + j] * urow[c];
             }
         }
         __syncthreads();
     }
 
-    // write back owned rows
     for (int e = tid; e < nrows * NB; e += blockDim.x) {
         int r = e / NB, c = e - r * NB;
         A[(int64_t)(rowbase + r) * lda + c] = blk[r * LDB + c];
     }
-
     __syncthreads();
     if (w == 0 && tid < 64) panel_net_moves(NB, kglob, s_piv, swapmap);
 }
-
 
 // ------------------------------------------------------ single-WG panel
 // Used when the whole M x NB panel fits one workgroup's LDS.  Column j is
@@ -525,6 +597,8 @@ template <typename T>
 static int launch_panel(la_ctx *ctx, T *Apanel, int64_t lda, int M, int NB, int kglob,
                         int64_t *d_ipiv, const PanelWS &pw, int32_t *d_status, uint64_t ticks)
 {
+    // algorithmic traffic: the panel is read and written once
+    ProfScope ps(ctx, LA_K_PANEL, (double)M * NB * NB, 2.0 * M * NB * sizeof(T));
     static bool attr_local[2] = {false, false};
     const int ai0 = sizeof(T) == 8 ? 0 : 1;
     const size_t local_bytes = (size_t)M * (NB + 1) * sizeof(T);
@@ -540,28 +614,30 @@ static int launch_panel(la_ctx *ctx, T *Apanel, int64_t lda, int M, int NB, int 
         return LA_OK;
     }
     int maxp = std::min(PANEL_MAXP, ctx->num_cus > 0 ? ctx->num_cus : 256);
-    int P = (int)std::min<int64_t>(maxp, ceil_div(M, 32));
-    if (P < 1) P = 1;
-    int R = (int)ceil_div(M, P);
-    P = (int)ceil_div(M, R);
-    size_t shmem = ((size_t)R * (NB + 1) + 2 * NB) * sizeof(T);
-    if (shmem > 150 * 1024)
-        return set_error(ctx, LA_ERR_ARG, "getrf panel: rows per workgroup exceed LDS");
-    static bool attr_set[2] = {false, false};
+    // rows per workgroup: 64 per lane-row of the first wavefront
+    int rpl = (int)ceil_div(ceil_div(M, maxp), 64);
+    if (rpl < 1) rpl = 1;
+    if (rpl > 4)
+        return set_error(ctx, LA_ERR_ARG, "getrf panel: more than 65536 rows per panel");
+    int R = 64 * rpl;
+    int P = (int)ceil_div(M, R);
+    size_t shmem = (size_t)R * (NB + 1) * sizeof(T);
+    static bool attr_set[2][4] = {{false, false, false, false}, {false, false, false, false}};
     const int ai = sizeof(T) == 8 ? 0 : 1;
-    if (!attr_set[ai]) {
-        hipFuncSetAttribute((const void *)k_getrf_panel<T>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024);
-        attr_set[ai] = true;
+    const void *fn = rpl == 1 ? (const void *)k_getrf_panel<T, 1>
+                   : rpl == 2 ? (const void *)k_getrf_panel<T, 2>
+                   : rpl == 3 ? (const void *)k_getrf_panel<T, 3>
+                              : (const void *)k_getrf_panel<T, 4>;
+    if (!attr_set[ai][rpl - 1]) {
+        hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)PANEL_LDS_MAX);
+        attr_set[ai][rpl - 1] = true;
     }
-    T *rowbuf = reinterpret_cast<T *>(pw.rowbuf);
-    T *diagbuf = reinterpret_cast<T *>(pw.diagbuf);
+    uint64_t *candp = pw.slots;
+    uint64_t *diagp = reinterpret_cast<uint64_t *>(pw.diagbuf);
     int64_t *swapmap = pw.swapmap;
-    uint64_t *slots = pw.slots;
-    void *args[] = {&Apanel, &lda, &M, &NB, &R, &kglob, &d_ipiv, &slots, &rowbuf,
-                    &diagbuf, &swapmap, &d_status, &ticks};
-    LA_HIP_CHECK(ctx, hipLaunchCooperativeKernel((const void *)k_getrf_panel<T>, dim3(P),
-                                                 dim3(256), args, (unsigned)shmem,
+    void *args[] = {&Apanel, &lda, &M, &NB, &R, &kglob, &d_ipiv, &candp, &diagp,
+                    &swapmap, &d_status, &ticks};
+    LA_HIP_CHECK(ctx, hipLaunchCooperativeKernel(fn, dim3(P), dim3(256), args, (unsigned)shmem,
                                                  ctx->stream));
     return LA_OK;
 }
@@ -572,6 +648,8 @@ static int launch_rowmoves(la_ctx *ctx, T *A, int64_t lda, int64_t n, int64_t c_
 {
     int64_t ncols = c_lo + (n - c_hi);
     int nblk = (int)ceil_div(ncols, 64);
+    // up to 2*NB_IN rows move: read + write of each moved row segment
+    ProfScope ps(ctx, LA_K_ROWMOVE, 0.0, 2.0 * 2 * NB_IN * ncols * sizeof(T));
     // one extra block permutes the perm vector
     hipLaunchKernelGGL(k_apply_rowmoves<T>, dim3(nblk + 1), dim3(256), 0, ctx->stream, A,
                        lda, (int)n, (int)c_lo, (int)c_hi, swapmap, perm);
@@ -584,6 +662,8 @@ static int launch_trsm(la_ctx *ctx, const T *L, int64_t ldl, int nb, T *B, int64
                        int64_t ncols)
 {
     if (ncols <= 0) return LA_OK;
+    ProfScope ps(ctx, LA_K_TRSM, (double)nb * nb * ncols,
+                 (double)sizeof(T) * (2.0 * nb * ncols + 0.5 * nb * nb));
     hipLaunchKernelGGL(k_trsm_unit_lower<T>, dim3((unsigned)ceil_div(ncols, 64)), dim3(256), 0,
                        ctx->stream, L, ldl, nb, B, ldb, (int)ncols);
     LA_HIP_CHECK(ctx, hipGetLastError());
@@ -600,9 +680,9 @@ int getrf_core(la_ctx *ctx, int64_t m, int64_t n, T *A, int64_t lda, int64_t *d_
         return set_error(ctx, LA_ERR_ARG, "getrf: extent too large");
     const int64_t kmax = m < n ? m : n;
     // panel workspace
-    size_t slot_bytes = (size_t)2 * PANEL_MAXP * 2 * sizeof(uint64_t);
-    size_t row_bytes = (size_t)2 * PANEL_MAXP * NB_IN * sizeof(T);
-    size_t diag_bytes = (size_t)2 * NB_IN * sizeof(T);
+    size_t slot_bytes = (size_t)2 * PANEL_MAXP * CAND_WORDS * sizeof(uint64_t);
+    size_t row_bytes = 256;
+    size_t diag_bytes = (size_t)2 * 2 * NB_IN * sizeof(uint64_t);
     size_t map_bytes = (size_t)(1 + 4 * NB_IN) * sizeof(int64_t);
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     size_t total = al(slot_bytes) + al(row_bytes) + al(diag_bytes) + al(map_bytes);
@@ -617,8 +697,11 @@ int getrf_core(la_ctx *ctx, int64_t m, int64_t n, T *A, int64_t lda, int64_t *d_
     pw.swapmap = (int64_t *)b;
 
     // reset slots: step tag 0xffffffff never occurs
-    const int64_t nslot_words = 2 * PANEL_MAXP * 2;
+    // step tag 0xffffffff never occurs: no stale word can validate
+    const int64_t nslot_words = 2 * PANEL_MAXP * CAND_WORDS;
     if ((st = fill_u64(ctx, pw.slots, nslot_words, 0xffffffff00000000ull))) return st;
+    if ((st = fill_u64(ctx, (uint64_t *)pw.diagbuf, 2 * 2 * NB_IN, 0xffffffff00000000ull)))
+        return st;
     if ((st = fill_i32(ctx, d_status, 1, 0))) return st;
     if (d_perm && (st = iota_u64(ctx, d_perm, m))) return st;
     const uint64_t ticks = spin_limit_ticks(ctx);

@@ -5,6 +5,7 @@
 #include <string>
 #include <vector>
 #include <algorithm>
+#include <type_traits>
 #include "../../include/la_hip.h"
 
 namespace lahip {
@@ -23,7 +24,7 @@ typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
 // exchange without the agent-scope acquire/release fences, which on gfx950
 // invalidate / write back the whole L2 (buffer_inv sc1 / buffer_wbl2 sc1).
 template <typename T>
-__device__ inline void st_dev(T *p, T v) {
+__device__ inline void st_dev(T *p, typename std::remove_reference<T>::type v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 template <typename T>
@@ -42,6 +43,18 @@ struct Workspace {
 
 }  // namespace lahip
 
+namespace lahip {
+struct ProfRecord {
+    hipEvent_t start, stop;
+    int kclass;
+    double flop, bytes;
+};
+struct ProfStats {
+    int64_t calls = 0;
+    double ms = 0, flop = 0, bytes = 0;
+};
+}  // namespace lahip
+
 struct la_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -54,6 +67,11 @@ struct la_ctx {
     int32_t *d_scratch = nullptr;   // 4 KiB of device status/counter words
     int32_t *h_pinned = nullptr;    // 4 KiB pinned host mirror
     std::string last_error;
+    // kernel timing harness
+    bool prof = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<lahip::ProfRecord> prof_pending;
+    lahip::ProfStats prof_stats[LA_K_COUNT];
 };
 
 namespace lahip {
@@ -81,6 +99,16 @@ int ws_get(la_ctx *ctx, int id, size_t bytes, void **out);
     } while (0)
 
 static inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Brackets the launches made during its lifetime with events when the
+// context's timing harness is on (see la_ctx_set_profiling).
+struct ProfScope {
+    la_ctx *ctx;
+    ProfRecord rec;
+    bool on;
+    ProfScope(la_ctx *c, int kclass, double flop, double bytes);
+    ~ProfScope();
+};
 
 // ---- helpers implemented in util.hip --------------------------------------
 uint64_t spin_limit_ticks(la_ctx *ctx);

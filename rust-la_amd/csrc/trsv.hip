@@ -158,6 +158,9 @@ static int launch_trsv_mode(la_ctx *ctx, int64_t n, const TF *Tm, int64_t ldt, i
     int32_t *ticket = flags + nblocks;
     int32_t *d_err = ctx->d_scratch;   // word 0: sticky device error
     const uint64_t ticks = spin_limit_ticks(ctx);
+    // algorithmic: one triangle of the factor + right-hand sides in and out
+    ProfScope ps(ctx, LA_K_TRSV, (double)n * n * nrhs,
+                 0.5 * n * n * sizeof(TF) + 2.0 * n * nrhs * 8.0);
     for (int64_t r0 = 0; r0 < nrhs; r0 += NR) {
         const int nr = (int)std::min<int64_t>(NR, nrhs - r0);
         hipLaunchKernelGGL(k_zero_i32, dim3((unsigned)ceil_div(nblocks + 1, 256)), dim3(256), 0,

@@ -30,6 +30,9 @@ LA_ERR_TIMEOUT = 10
 
 PREC_FP64, PREC_FP32, PREC_BF16, PREC_FP16, PREC_FP32X3 = 0, 1, 2, 3, 4
 
+KERNEL_CLASSES = ["gemm64", "gemm32", "gemm16", "panel", "trsv", "resid", "trsm", "rowmove",
+                  "batch"]
+
 _dp = ctypes.POINTER(ctypes.c_double)
 _u64p = ctypes.POINTER(ctypes.c_uint64)
 _i32p = ctypes.POINTER(ctypes.c_int32)
@@ -91,6 +94,9 @@ def load():
     L.la_dgetrs_dev.argtypes = [_vp, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _vp]
     L.la_dpotrf_dev.argtypes = [_vp, _i64, _vp, _i64, _vp]
     L.la_dpotrs_dev.argtypes = [_vp, _i64, _vp, _i64, _i64, _vp, _i64, _vp, _i64]
+    L.la_ctx_set_profiling.argtypes = [_vp, ctypes.c_int]
+    L.la_ctx_reset_kernel_stats.argtypes = [_vp]
+    L.la_ctx_kernel_stats.argtypes = [_vp, ctypes.c_int, _i64p, _dp, _dp, _dp]
     L.la_fill_uniform_dev.argtypes = [_vp, ctypes.c_uint64, ctypes.c_uint64, _i64, _vp]
     L.la_fill_normal_dev.argtypes = [_vp, ctypes.c_uint64, ctypes.c_uint64, _i64, _vp]
     for name, args in _OPTIONAL.items():
@@ -166,6 +172,33 @@ class Context:
 
     def stream(self):
         return self.lib.la_ctx_stream(self.h)
+
+    def set_stream(self, hip_stream):
+        """Run the library on an existing stream (e.g. torch's current
+        stream, so torch-side copies and library kernels stay ordered)."""
+        self._check(self.lib.la_ctx_set_stream(self.h, ctypes.c_void_p(hip_stream)), "set_stream")
+
+    def use_torch_stream(self):
+        import torch
+        self.set_stream(torch.cuda.current_stream().cuda_stream)
+
+    def set_profiling(self, on):
+        self._check(self.lib.la_ctx_set_profiling(self.h, 1 if on else 0), "set_profiling")
+
+    def reset_kernel_stats(self):
+        self._check(self.lib.la_ctx_reset_kernel_stats(self.h), "reset_kernel_stats")
+
+    def kernel_stats(self):
+        """{class: (calls, total_ms, algorithmic flop, algorithmic bytes)}"""
+        out = {}
+        for i, name in enumerate(KERNEL_CLASSES):
+            c, ms, fl, by = _i64(), ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+            self._check(self.lib.la_ctx_kernel_stats(self.h, i, ctypes.byref(c), ctypes.byref(ms),
+                                                     ctypes.byref(fl), ctypes.byref(by)),
+                        "kernel_stats")
+            if c.value:
+                out[name] = (c.value, ms.value, fl.value, by.value)
+        return out
 
     def device_info(self):
         cus, hbm, clk = ctypes.c_int(), _i64(), ctypes.c_int()

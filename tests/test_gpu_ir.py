@@ -1,0 +1,110 @@
+"""GPU tests of the mixed-precision solver and the seeded generator.
+
+The generator is pinned against the oracle's numpy restatement; the FP32
+factorisation with FP64 refinement is pinned by closed forms: for
+A = U diag(sigma) V^T the exact solution is x* = V diag(1/sigma) U^T b,
+and the normwise backward error must reach the requested tolerance."""
+import numpy as np
+import pytest
+
+import oracle
+import la_hip
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def tctx(torch_cuda):
+    c = la_hip.Context(0)
+    c.use_torch_stream()
+    yield c
+    c.close()
+
+
+def test_generator_matches_oracle(tctx, torch_cuda):
+    torch = torch_cuda
+    n, kappa, nrefl, seed = 200, 1e4, 8, 7
+    A = torch.empty((n, n), dtype=torch.float64, device="cuda")
+    b = torch.empty((n, 2), dtype=torch.float64, device="cuda")
+    xs = torch.empty_like(b)
+    tctx.gen_svd_matrix_dev(n, kappa, nrefl, seed, A, b=b, xstar=xs)
+    torch.cuda.synchronize()
+    a_o, us, vs, sig = oracle.householder_svd_matrix(n, kappa, nrefl, seed)
+    b_o = oracle.normal(seed, 300, (n, 2))
+    x_o = oracle.householder_exact_solution(us, vs, sig, b_o)
+    assert np.abs(A.cpu().numpy() - a_o).max() < 1e-13
+    assert np.abs(b.cpu().numpy() - b_o).max() < 1e-13
+    assert np.abs(xs.cpu().numpy() - x_o).max() / np.abs(x_o).max() < 1e-12
+
+
+@pytest.mark.parametrize("kappa", [1e2, 1e4, 1e6])
+def test_fp32_ir_converges(tctx, torch_cuda, kappa):
+    torch = torch_cuda
+    n = 1000
+    A = torch.empty((n, n), dtype=torch.float64, device="cuda")
+    b = torch.empty((n, 1), dtype=torch.float64, device="cuda")
+    xs = torch.empty_like(b)
+    tctx.gen_svd_matrix_dev(n, kappa, 16, 11, A, b=b, xstar=xs)
+    X = torch.empty_like(b)
+    tol = 1e-15 * n
+    rep = tctx.gesv_ir_dev(A, b, X, la_hip.PREC_FP32, tol, 30)
+    assert rep.status == la_hip.LA_OK, (rep.status, rep.backward_error)
+    assert rep.backward_error <= tol
+    # independent check of eta and of the forward error against x*
+    eta = tctx.backward_error_dev(A, X, b)
+    assert eta <= tol
+    # forward error is bounded by kappa times the backward error reached
+    fwd = ((X - xs).abs().max() / xs.abs().max()).item()
+    assert fwd <= 10 * kappa * max(eta, 1.1e-16)
+    assert rep.iterations <= 12
+    # refined to the FP64 floor, the solution agrees with x* to ~kappa*u
+    rep2 = tctx.gesv_ir_dev(A, b, X, la_hip.PREC_FP32, 4e-16, 40)
+    fwd2 = ((X - xs).abs().max() / xs.abs().max()).item()
+    assert fwd2 <= 50 * kappa * 1.1e-16, (fwd2, rep2.backward_error, rep2.iterations)
+
+
+@pytest.mark.parametrize("nrhs", [1, 3, 16])
+def test_fp32_ir_multi_rhs(tctx, torch_cuda, nrhs):
+    torch = torch_cuda
+    n = 777
+    A = torch.empty((n, n), dtype=torch.float64, device="cuda")
+    b = torch.empty((n, nrhs), dtype=torch.float64, device="cuda")
+    xs = torch.empty_like(b)
+    tctx.gen_svd_matrix_dev(n, 1e3, 8, 5, A, b=b, xstar=xs)
+    X = torch.empty_like(b)
+    rep = tctx.gesv_ir_dev(A, b, X, la_hip.PREC_FP32, 1e-16 * n, 30)
+    assert rep.status == la_hip.LA_OK
+    eta = tctx.backward_error_dev(A, X, b)
+    assert eta <= 1e-16 * n
+    assert ((X - xs).abs().max() / xs.abs().max()).item() < 10 * 1e3 * eta
+
+
+def test_fp64_full_size_cfg2(tctx, torch_cuda):
+    """Config 2 at full size (n = 8192, 16 right-hand sides, normal entries):
+    size-independent properties - pivot vector is a permutation, P A = L U
+    on random probes, normwise backward error of the solve."""
+    torch = torch_cuda
+    n = 8192
+    A = torch.empty((n, n), dtype=torch.float64, device="cuda")
+    tctx.fill_normal_dev(2024, 0, A)
+    B = torch.empty((n, 16), dtype=torch.float64, device="cuda")
+    tctx.fill_normal_dev(2024, 2, B)
+    W = A.clone()
+    piv = torch.empty(n, dtype=torch.int64, device="cuda")
+    info = torch.zeros(4, dtype=torch.int32, device="cuda")
+    tctx.getrf_dev(W, piv, info)
+    X = torch.empty_like(B)
+    st = torch.zeros(1, dtype=torch.int32, device="cuda")
+    tctx.getrs_dev(W, piv, B, X, st)
+    torch.cuda.synchronize()
+    assert info[1].item() == 0 and st.item() == 0
+    p = piv.cpu()
+    assert bool((p.sort().values == torch.arange(n)).all())
+    z = torch.randn(n, 2, dtype=torch.float64, device="cuda")
+    L = torch.tril(W, -1) + torch.eye(n, dtype=torch.float64, device="cuda")
+    U = torch.triu(W)
+    r = A[piv] @ z - L @ (U @ z)
+    # classical bound ||PA - LU|| <= gamma_n ||L|| ||U||; n*u = 9.1e-13 here
+    assert (r.abs().max() / (A.abs().sum(1).max() * z.abs().max())).item() < 2e-13
+    eta = tctx.backward_error_dev(A, X, B)
+    assert eta < 1e-14
