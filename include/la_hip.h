@@ -200,6 +200,44 @@ int la_dgesv_ir_rm(la_ctx *ctx, int64_t n, int64_t nrhs, const double *A,
                    const double *B, double *X, int prec, double tol, int max_iter,
                    la_ir_report *report);
 
+/* ------------------------------------------------- batched uneven solvers
+ * Problem i has order n[i]; its matrix starts at A + offA[i] (row-major,
+ * ld = n[i]) and its right-hand sides at B + offB[i] (n[i] x nrhs[i],
+ * row-major).  All arrays are device pointers.  The whole batch runs as ONE
+ * persistent kernel launch (largest problems first from an atomic queue),
+ * replayed from a captured hipGraph when the plan is reused. */
+
+typedef struct la_batch_plan la_batch_plan;
+
+/* Build a plan for a batch (host copies of the sizes are required to sort
+ * and bucket the problems).  kind: 0 = Cholesky solve (FP64),
+ * 1 = LU-IR solve (prec = la_prec).  The plan owns its device queue and
+ * graph. */
+int la_batch_plan_create(la_ctx *ctx, int kind, int64_t count,
+                         const int32_t *h_n, const int32_t *h_nrhs, int prec,
+                         la_batch_plan **out);
+void la_batch_plan_destroy(la_batch_plan *plan);
+/* Human-readable tile choice of the plan (autotuned or default). */
+const char *la_batch_plan_describe(la_batch_plan *plan);
+
+/* Batched Cholesky + two triangular solves, FP64:
+ * L_i = chol(A_i) written to L + offA[i] (full, upper zeroed), X_i solves
+ * A_i X_i = B_i.  status[i] gets LA_OK / LA_NOT_SYMMETRIC / LA_NOT_SPD per
+ * problem; a failing problem never affects its neighbours.  No reference
+ * counterpart as a batch: each problem is CholeskyDecomposition::new + solve
+ * (src/decomp/cholesky.rs:56-144). */
+int la_dpotrf_potrs_batched_dev(la_ctx *ctx, la_batch_plan *plan,
+                                const int64_t *d_offA, const int64_t *d_offB,
+                                const double *A, double *L,
+                                const double *B, double *X, int32_t *d_status);
+
+/* Run the tile autotuner for `kind` over the size buckets of [nmin, nmax]
+ * and store the winners in `cache_path` (JSON); later plans read it.
+ * cache_path may be NULL to tune in memory only. */
+int la_autotune(la_ctx *ctx, int kind, int nmin, int nmax, const char *cache_path);
+/* Load a previously written autotune cache. */
+int la_autotune_load(la_ctx *ctx, const char *cache_path);
+
 /* ------------------------------------------------------- test generators
  * Seeded counter-based inputs (SplitMix64 finaliser over a counter).
  * Element i of stream s under seed q is mix(K + (i+1)*G) with
@@ -220,6 +258,13 @@ int la_gen_svd_matrix_dev(la_ctx *ctx, int64_t n, double kappa, int nrefl,
                           uint64_t seed, uint64_t u_stream, uint64_t v_stream,
                           double *A, int64_t lda,
                           int64_t nrhs, uint64_t b_stream, double *b, double *xstar);
+
+/* Batch of SPD test matrices A_i = G_i G_i^T / n_i + I, G_i standard normal
+ * from stream stream_base + i (row-major, A_i at A + offA[i]).  G_scratch
+ * must have the same layout and size as A.  All pointers are device. */
+int la_gen_spd_batch_dev(la_ctx *ctx, int64_t count, const int32_t *d_n,
+                         const int64_t *d_offA, uint64_t seed, uint64_t stream_base,
+                         double *A, double *G_scratch);
 
 /* ------------------------------------------------------------ utilities */
 /* eta = ||B - A X||_inf / (||A||_inf ||X||_inf + ||B||_inf), FP64 on device. */

@@ -1,0 +1,248 @@
+// Batched uneven solvers: plans, graph capture and the tile autotuner.
+//
+// A plan sorts the problems by order (largest first, so the persistent
+// workgroups finish together), uploads the order and sizes once, owns the
+// device queue counters, and records the solver launch in a hipGraph the
+// first time it runs with a given set of buffers; later calls with the same
+// buffers replay the graph.  The launch is a single persistent kernel; the
+// queue is reset by the kernel itself, so the graph has no memset node.
+#include "la_common.hpp"
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+namespace lahip {
+
+// kernels (batched_chol.hip / batched_lu.hip)
+int launch_chol_batched(la_ctx *ctx, hipStream_t s, int grid, int count, const int32_t *order,
+                        const int32_t *nvec, const int64_t *offA, const int64_t *offB,
+                        const double *A, double *L, const double *B, double *X, int32_t *status,
+                        uint32_t *queue, const int nb[4]);
+int chol_batched_grid(la_ctx *ctx);
+int fill_spd_batch(la_ctx *ctx, int count, const int32_t *nvec, const int64_t *offA, double *A);
+
+}  // namespace lahip
+
+using namespace lahip;
+
+struct la_batch_plan {
+    la_ctx *ctx = nullptr;
+    int kind = 0;
+    int64_t count = 0;
+    int prec = 0;
+    int32_t *d_order = nullptr, *d_n = nullptr, *d_nrhs = nullptr;
+    uint32_t *d_queue = nullptr;
+    int nb[4] = {16, 16, 32, 32};
+    int grid = 0;
+    std::string desc;
+    hipStream_t cap_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    const void *key[8] = {nullptr};
+};
+
+static const int kCholCandidates[] = {8, 16, 32};
+
+static void describe(la_batch_plan *p) {
+    char buf[256];
+    snprintf(buf, sizeof buf,
+             "kind=%s count=%lld grid=%d panel widths by bucket [<=32, <=64, <=128, <=256] = "
+             "%d/%d/%d/%d%s",
+             p->kind == 0 ? "cholesky" : "lu-ir", (long long)p->count, p->grid, p->nb[0],
+             p->nb[1], p->nb[2], p->nb[3], p->ctx->chol_tuned ? " (autotuned)" : " (defaults)");
+    p->desc = buf;
+}
+
+extern "C" int la_batch_plan_create(la_ctx *ctx, int kind, int64_t count, const int32_t *h_n,
+                                    const int32_t *h_nrhs, int prec, la_batch_plan **out)
+{
+    if (!ctx || !out || !h_n || count <= 0 || count > INT32_MAX) return LA_ERR_ARG;
+    if (kind != 0) return set_error(ctx, LA_ERR_ARG, "batch plan: kind not supported");
+    *out = nullptr;
+    for (int64_t i = 0; i < count; ++i)
+        if (h_n[i] <= 0 || h_n[i] > 256)
+            return set_error(ctx, LA_ERR_ARG, "batched Cholesky: order must be in [1, 256]");
+    if (h_nrhs)
+        for (int64_t i = 0; i < count; ++i)
+            if (h_nrhs[i] != 1) return set_error(ctx, LA_ERR_ARG, "batched Cholesky: nrhs must be 1");
+    la_batch_plan *p = new la_batch_plan();
+    p->ctx = ctx;
+    p->kind = kind;
+    p->count = count;
+    p->prec = prec;
+    std::vector<int32_t> order(count);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int32_t a, int32_t b) { return h_n[a] > h_n[b]; });
+    auto fail = [&](hipError_t e, const char *w) {
+        la_batch_plan_destroy(p);
+        return hip_fail(ctx, e, w);
+    };
+    hipError_t e;
+    if ((e = hipMalloc(&p->d_order, count * 4)) != hipSuccess) return fail(e, "plan alloc");
+    if ((e = hipMalloc(&p->d_n, count * 4)) != hipSuccess) return fail(e, "plan alloc");
+    if ((e = hipMalloc(&p->d_queue, 256)) != hipSuccess) return fail(e, "plan alloc");
+    if ((e = hipMemcpy(p->d_order, order.data(), count * 4, hipMemcpyHostToDevice)) != hipSuccess)
+        return fail(e, "plan upload");
+    if ((e = hipMemcpy(p->d_n, h_n, count * 4, hipMemcpyHostToDevice)) != hipSuccess)
+        return fail(e, "plan upload");
+    if ((e = hipMemset(p->d_queue, 0, 256)) != hipSuccess) return fail(e, "plan queue");
+    if ((e = hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking)) != hipSuccess)
+        return fail(e, "plan stream");
+    if (ctx->chol_tuned) memcpy(p->nb, ctx->chol_nb, sizeof p->nb);
+    p->grid = chol_batched_grid(ctx);
+    describe(p);
+    *out = p;
+    return LA_OK;
+}
+
+extern "C" void la_batch_plan_destroy(la_batch_plan *p)
+{
+    if (!p) return;
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
+    if (p->d_order) (void)hipFree(p->d_order);
+    if (p->d_n) (void)hipFree(p->d_n);
+    if (p->d_nrhs) (void)hipFree(p->d_nrhs);
+    if (p->d_queue) (void)hipFree(p->d_queue);
+    delete p;
+}
+
+extern "C" const char *la_batch_plan_describe(la_batch_plan *p)
+{
+    return p ? p->desc.c_str() : "null plan";
+}
+
+extern "C" int la_dpotrf_potrs_batched_dev(la_ctx *ctx, la_batch_plan *p, const int64_t *d_offA,
+                                           const int64_t *d_offB, const double *A, double *L,
+                                           const double *B, double *X, int32_t *d_status)
+{
+    if (!ctx || !p || !d_offA || !d_offB || !A || !L || !B || !X || !d_status) return LA_ERR_ARG;
+    if (p->kind != 0 || p->ctx != ctx) return LA_ERR_ARG;
+    const void *key[8] = {d_offA, d_offB, A, L, B, X, d_status, nullptr};
+    const bool same = p->exec && memcmp(key, p->key, sizeof key) == 0;
+    ProfScope ps(ctx, LA_K_BATCH, 0.0, 0.0);
+    if (!same) {
+        if (p->exec) { (void)hipGraphExecDestroy(p->exec); p->exec = nullptr; }
+        if (p->graph) { (void)hipGraphDestroy(p->graph); p->graph = nullptr; }
+        LA_HIP_CHECK(ctx, hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeThreadLocal));
+        int st = launch_chol_batched(ctx, p->cap_stream, p->grid, (int)p->count, p->d_order,
+                                     p->d_n, d_offA, d_offB, A, L, B, X, d_status, p->d_queue,
+                                     p->nb);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(p->cap_stream, &g);
+        if (st) { if (g) (void)hipGraphDestroy(g); return st; }
+        if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamEndCapture");
+        p->graph = g;
+        LA_HIP_CHECK(ctx, hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0));
+        memcpy(p->key, key, sizeof key);
+    }
+    LA_HIP_CHECK(ctx, hipGraphLaunch(p->exec, ctx->stream));
+    return LA_OK;
+}
+
+// ---------------------------------------------------------------- autotune
+// For each size bucket, time every candidate panel width on a synthetic
+// batch of that bucket's sizes and keep the fastest.
+extern "C" int la_autotune(la_ctx *ctx, int kind, int nmin, int nmax, const char *cache_path)
+{
+    if (!ctx || kind != 0 || nmin < 1 || nmax > 256 || nmin > nmax) return LA_ERR_ARG;
+    const int lo[4] = {1, 33, 65, 129}, hi[4] = {32, 64, 128, 256};
+    int best[4];
+    memcpy(best, ctx->chol_tuned ? ctx->chol_nb : (const int[4]){16, 16, 32, 32}, sizeof best);
+    hipEvent_t e0, e1;
+    LA_HIP_CHECK(ctx, hipEventCreate(&e0));
+    LA_HIP_CHECK(ctx, hipEventCreate(&e1));
+    int st = LA_OK;
+    for (int b = 0; b < 4 && st == LA_OK; ++b) {
+        const int a = std::max(lo[b], nmin), z = std::min(hi[b], nmax);
+        if (a > z) continue;
+        const int count = 2048;
+        std::vector<int32_t> hn(count);
+        std::vector<int64_t> ha(count), hb(count);
+        uint64_t s = 0x243F6A8885A308D3ull + b;
+        int64_t oa = 0, ob = 0;
+        for (int i = 0; i < count; ++i) {
+            s ^= s << 13; s ^= s >> 7; s ^= s << 17;
+            hn[i] = a + (int)(s % (uint64_t)(z - a + 1));
+            ha[i] = oa; hb[i] = ob;
+            oa += (int64_t)hn[i] * hn[i];
+            ob += hn[i];
+        }
+        la_batch_plan *p = nullptr;
+        if ((st = la_batch_plan_create(ctx, 0, count, hn.data(), nullptr, 0, &p))) break;
+        double *dA = nullptr, *dL = nullptr, *dB = nullptr, *dX = nullptr;
+        int64_t *dOA = nullptr, *dOB = nullptr;
+        int32_t *dS = nullptr;
+        hipError_t he = hipSuccess;
+        he = hipMalloc(&dA, oa * 8); if (he == hipSuccess) he = hipMalloc(&dL, oa * 8);
+        if (he == hipSuccess) he = hipMalloc(&dB, ob * 8);
+        if (he == hipSuccess) he = hipMalloc(&dX, ob * 8);
+        if (he == hipSuccess) he = hipMalloc(&dOA, count * 8);
+        if (he == hipSuccess) he = hipMalloc(&dOB, count * 8);
+        if (he == hipSuccess) he = hipMalloc(&dS, count * 4);
+        if (he == hipSuccess) he = hipMemcpy(dOA, ha.data(), count * 8, hipMemcpyHostToDevice);
+        if (he == hipSuccess) he = hipMemcpy(dOB, hb.data(), count * 8, hipMemcpyHostToDevice);
+        if (he != hipSuccess) st = hip_fail(ctx, he, "autotune alloc");
+        if (st == LA_OK) st = fill_spd_batch(ctx, count, p->d_n, dOA, dA);
+        if (st == LA_OK) st = la_fill_uniform_dev(ctx, 1, 1, ob, dB);
+        float bestms = 1e30f;
+        for (int cand : kCholCandidates) {
+            if (st) break;
+            for (int q = 0; q < 4; ++q) p->nb[q] = cand;
+            float ms = 0.f;
+            for (int rep = 0; rep < 3 && st == LA_OK; ++rep) {
+                (void)hipEventRecord(e0, ctx->stream);
+                st = launch_chol_batched(ctx, ctx->stream, p->grid, count, p->d_order, p->d_n, dOA,
+                                         dOB, dA, dL, dB, dX, dS, p->d_queue, p->nb);
+                (void)hipEventRecord(e1, ctx->stream);
+                (void)hipEventSynchronize(e1);
+                float t = 0.f;
+                (void)hipEventElapsedTime(&t, e0, e1);
+                if (rep > 0) ms += t;
+            }
+            if (st == LA_OK && ms < bestms) { bestms = ms; best[b] = cand; }
+        }
+        (void)hipFree(dA); (void)hipFree(dL); (void)hipFree(dB); (void)hipFree(dX);
+        (void)hipFree(dOA); (void)hipFree(dOB); (void)hipFree(dS);
+        la_batch_plan_destroy(p);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (st) return st;
+    memcpy(ctx->chol_nb, best, sizeof best);
+    ctx->chol_tuned = true;
+    if (cache_path) {
+        FILE *f = fopen(cache_path, "w");
+        if (!f) return set_error(ctx, LA_ERR_ARG, "autotune: cannot write cache");
+        fprintf(f, "{\"kind\": \"cholesky\", \"buckets\": [32, 64, 128, 256], \"panel_width\": [%d, %d, %d, %d]}\n",
+                best[0], best[1], best[2], best[3]);
+        fclose(f);
+    }
+    return LA_OK;
+}
+
+extern "C" int la_autotune_load(la_ctx *ctx, const char *cache_path)
+{
+    if (!ctx || !cache_path) return LA_ERR_ARG;
+    FILE *f = fopen(cache_path, "r");
+    if (!f) return set_error(ctx, LA_ERR_ARG, "autotune: cannot read cache");
+    char buf[512] = {0};
+    size_t nread = fread(buf, 1, sizeof buf - 1, f);
+    fclose(f);
+    buf[nread] = 0;
+    const char *k = strstr(buf, "\"panel_width\"");
+    int v[4];
+    if (!k || sscanf(strchr(k, '['), "[%d, %d, %d, %d]", &v[0], &v[1], &v[2], &v[3]) != 4)
+        return set_error(ctx, LA_ERR_ARG, "autotune: malformed cache");
+    for (int i = 0; i < 4; ++i)
+        if (v[i] != 8 && v[i] != 16 && v[i] != 32)
+            return set_error(ctx, LA_ERR_ARG, "autotune: bad panel width in cache");
+    memcpy(ctx->chol_nb, v, sizeof v);
+    ctx->chol_tuned = true;
+    return LA_OK;
+}

@@ -102,9 +102,72 @@ __global__ void k_gen_scale_rows(int64_t n, int nrhs, double kappa, double *y) {
     y[e] = y[e] / s;
 }
 
+// Batch of SPD matrices A_i = G_i G_i^T / n_i + I with G_i = normal(seed,
+// stream_base + i) (n_i x n_i, row-major): one workgroup per problem draws
+// G_i into the scratch slot (same layout as A), then forms the product tile
+// by tile through LDS and writes both triangles of A_i.
+__global__ __launch_bounds__(256) void k_gen_spd_batch(int count, const int32_t *nvec,
+                                                       const int64_t *offA, uint64_t seed,
+                                                       uint64_t stream_base, double *A, double *G)
+{
+    const int prob = blockIdx.x;
+    if (prob >= count) return;
+    const int n = nvec[prob];
+    double *Gp = G + offA[prob];
+    const uint64_t key = mix64(seed + (stream_base + prob + 1) * GOLDEN);
+    const double two_pi = 6.283185307179586476925286766559;
+    const int64_t cnt = (int64_t)n * n;
+    for (int64_t p = threadIdx.x; 2 * p < cnt; p += blockDim.x) {
+        double u1 = (double)((mix64(key + (uint64_t)(2 * p + 1) * GOLDEN) >> 11) + 1) * 0x1.0p-53;
+        double u2 = (double)(mix64(key + (uint64_t)(2 * p + 2) * GOLDEN) >> 11) * 0x1.0p-53;
+        double r = sqrt(-2.0 * log(u1));
+        Gp[2 * p] = r * cos(two_pi * u2);
+        if (2 * p + 1 < cnt) Gp[2 * p + 1] = r * sin(two_pi * u2);
+    }
+    __syncthreads();
+    __shared__ double gi[32][33], gj[32][33];
+    const double inv_n = 1.0 / (double)n;
+    for (int i0 = 0; i0 < n; i0 += 32)
+        for (int j0 = 0; j0 <= i0; j0 += 32) {
+            double acc[4] = {0, 0, 0, 0};
+            const int tr = threadIdx.x >> 5, tc = threadIdx.x & 31;   // 8 x 32 threads
+            for (int k0 = 0; k0 < n; k0 += 32) {
+                for (int e = threadIdx.x; e < 32 * 32; e += 256) {
+                    const int r = e >> 5, c = e & 31;
+                    gi[r][c] = (i0 + r < n && k0 + c < n) ? Gp[(int64_t)(i0 + r) * n + k0 + c] : 0.0;
+                    gj[r][c] = (j0 + r < n && k0 + c < n) ? Gp[(int64_t)(j0 + r) * n + k0 + c] : 0.0;
+                }
+                __syncthreads();
+                for (int q = 0; q < 4; ++q)
+                    for (int k = 0; k < 32; ++k) acc[q] += gi[tr + 8 * q][k] * gj[tc][k];
+                __syncthreads();
+            }
+            for (int q = 0; q < 4; ++q) {
+                const int r = i0 + tr + 8 * q, c = j0 + tc;
+                if (r < n && c < n && c <= r) {
+                    const double v = acc[q] * inv_n + (r == c ? 1.0 : 0.0);
+                    A[offA[prob] + (int64_t)r * n + c] = v;
+                    A[offA[prob] + (int64_t)c * n + r] = v;
+                }
+            }
+        }
+}
+
 }  // namespace lahip
 
 using namespace lahip;
+
+extern "C" int la_gen_spd_batch_dev(la_ctx *ctx, int64_t count, const int32_t *d_n,
+                                    const int64_t *d_offA, uint64_t seed, uint64_t stream_base,
+                                    double *A, double *G_scratch)
+{
+    if (!ctx || !d_n || !d_offA || !A || !G_scratch || count <= 0 || count > INT32_MAX)
+        return LA_ERR_ARG;
+    hipLaunchKernelGGL(k_gen_spd_batch, dim3((unsigned)count), dim3(256), 0, ctx->stream,
+                       (int)count, d_n, d_offA, seed, stream_base, A, G_scratch);
+    LA_HIP_CHECK(ctx, hipGetLastError());
+    return LA_OK;
+}
 
 extern "C" int la_gen_svd_matrix_dev(la_ctx *ctx, int64_t n, double kappa, int nrefl,
                                      uint64_t seed, uint64_t u_stream, uint64_t v_stream,

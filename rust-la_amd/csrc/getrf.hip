@@ -139,6 +139,37 @@ __device__ void panel_net_moves(int NB, int kglob, const int *s_piv, int64_t *sw
     if (lane == 0) swapmap[0] = na + __popcll(bal_b);
 }
 
+
+// X = L^-1 for the unit lower nb x nb triangle L (LDS, row stride ldl; only
+// the strict lower part is read).  Row sweep: X[i][c] = -sum_{t=c}^{i-1}
+// L[i][t] X[t][c] for c < i, X[i][i] = 1; the 256 threads split each dot
+// four ways.  X (LDS, 64 x 65) is written to out (NB_IN x NB_IN, zero above
+// the diagonal).  Called by a whole workgroup.
+template <typename T>
+__device__ void invert_unit_lower(const T *L, int ldl, int nb, T (*X)[65], T *out)
+{
+    __shared__ T part[4][64];
+    const int tid = threadIdx.x, c = tid & 63, q = tid >> 6;
+    for (int e = tid; e < 64 * 64; e += blockDim.x) {
+        const int r = e >> 6, cc = e & 63;
+        X[r][cc] = (r == cc) ? T(1) : T(0);
+    }
+    __syncthreads();
+    for (int i = 1; i < nb; ++i) {
+        T s = T(0);
+        if (c < i)
+            for (int t = c + q; t < i; t += 4) s += L[i * ldl + t] * X[t][c];
+        part[q][c] = s;
+        __syncthreads();
+        if (q == 0 && c < i) X[i][c] = -((part[0][c] + part[1][c]) + (part[2][c] + part[3][c]));
+        __syncthreads();
+    }
+    for (int e = tid; e < NB_IN * NB_IN; e += blockDim.x) {
+        const int r = e / NB_IN, cc = e - r * NB_IN;
+        out[e] = (r < nb && cc < nb) ? X[r][cc] : T(0);
+    }
+}
+
 // ------------------------------------------------------------- panel kernel
 // Factors the M x NB panel whose top-left element is A (row-major, lda),
 // global column/row offset kglob, with P cooperative workgroups; workgroup
@@ -176,24 +207,41 @@ __device__ inline void put_value(uint64_t *w, uint64_t tag32, float v) {
     st_dev(w, tag32 | (uint64_t)__float_as_uint(v));
 }
 
+#ifdef LAHIP_PANEL_TIMING
+// Phase clock of workgroup 0 (wall-clock ticks summed over columns):
+// [0] gather slots, [1] winner fetch, [2] swap, [3] critical wave, [4] bulk
+__device__ unsigned long long g_panel_phase[8];
+#define PHASE_MARK(k)                                                          \
+    do {                                                                       \
+        if (w == 0 && tid == 0) {                                              \
+            const uint64_t now_ = wall_clock64();                              \
+            atomicAdd(&g_panel_phase[k], (unsigned long long)(now_ - ph_t));   \
+            ph_t = now_;                                                       \
+        }                                                                      \
+    } while (0)
+#else
+#define PHASE_MARK(k) do {} while (0)
+#endif
+
 template <typename T, int RPL>
 __global__ __launch_bounds__(256)
 void k_getrf_panel(T *A, int64_t lda, int M, int NB, int R, int kglob, int64_t *ipiv,
                    uint64_t *cand, uint64_t *diagw, int64_t *swapmap, int32_t *status,
-                   uint64_t spin_ticks)
+                   uint64_t spin_ticks, T *linv)
 {
     constexpr int W = PanelWords<T>::W;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int LDB = NB + 1;
     T *blk = reinterpret_cast<T *>(smem_raw);
-    __shared__ __align__(16) uint32_t s_urow[W * NB_IN];   // winner row (raw halves)
+    // winner row (raw halves), double-buffered by step parity: the bulk
+    // update of step j-1 reads one buffer while step j's row lands in the other
+    __shared__ __align__(16) uint32_t s_urow[2][W * NB_IN];
     __shared__ __align__(16) uint32_t s_drow[W * NB_IN];   // old row j (raw halves)
     __shared__ uint64_t s_key[4];
     __shared__ int s_idx[4];
     __shared__ int s_abort;
     __shared__ int s_piv[NB_IN];
     __shared__ int s_skip[2];
-    const T *urow = reinterpret_cast<const T *>(s_urow);
     const T *drow = reinterpret_cast<const T *>(s_drow);
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -254,95 +302,155 @@ void k_getrf_panel(T *A, int64_t lda, int M, int NB, int R, int kglob, int64_t *
         local_cand(0, key, idx, neg);
         publish(0, key, idx, neg);
     }
-
+    __shared__ int s_p;
+    int pend_j = -1;          // step whose bulk update is still pending
+#ifdef LAHIP_PANEL_TIMING
+    uint64_t ph_t = wall_clock64();
+#endif
     for (int j = 0; j < NB; ++j) {
         const uint32_t gs = (uint32_t)(kglob + j);
         const int buf = gs & 1;
         const uint64_t tagb = ((gs >> 1) & 1) ? TAG_BIT : 0;
+        const T *urow = reinterpret_cast<const T *>(s_urow[j & 1]);
 
-        // 1. gather all P candidates of step j
-        uint64_t ck = 0;
-        int ci = IDX_NONE;
-        if (tid < P) {
-            const uint64_t *slot = cand + ((size_t)buf * PANEL_MAXP + tid) * CAND_WORDS;
-            const uint64_t t0 = wall_clock64();
-            while (true) {
-                const uint64_t w1 = ld_dev(slot + 1);
-                const uint64_t w0 = ld_dev(slot + 0);
-                if ((uint32_t)(w1 >> 32) == gs && (w0 & TAG_BIT) == tagb) {
-                    ck = w0 & ~TAG_BIT;
-                    ci = (int)((uint32_t)w1 & 0x7fffffffu);
-                    break;
+        if (wave == 0) {
+            // 1. gather the P candidates of step j (lane q polls workgroup q)
+            uint64_t ck = 0;
+            int ci = IDX_NONE;
+            int abort = 0;
+            for (int q = lane; q < P; q += 64) {
+                const uint64_t *slot = cand + ((size_t)buf * PANEL_MAXP + q) * CAND_WORDS;
+                const uint64_t t0 = wall_clock64();
+                for (uint32_t spin = 0;; ++spin) {
+                    const uint64_t w1 = ld_dev(slot + 1);
+                    const uint64_t w0 = ld_dev(slot + 0);
+                    if ((uint32_t)(w1 >> 32) == gs && (w0 & TAG_BIT) == tagb) {
+                        const uint64_t k = w0 & ~TAG_BIT;
+                        const int i = (int)((uint32_t)w1 & 0x7fffffffu);
+                        if (better(k, i, ck, ci)) { ck = k; ci = i; }
+                        break;
+                    }
+                    if ((spin & 31) == 31 && ld_status(status) != 0) { abort = 1; break; }
+                    if (wall_clock64() - t0 > spin_ticks) {
+                        atomicCAS((int *)status, 0, (int)LA_ERR_TIMEOUT);
+                        abort = 1;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
                 }
-                if (ld_status(status) != 0) { s_abort = 1; break; }
-                if (wall_clock64() - t0 > spin_ticks) {
-                    atomicCAS((int *)status, 0, (int)LA_ERR_TIMEOUT);
-                    s_abort = 1;
-                    break;
+            }
+            wave_argmax(ck, ci);
+            abort = __any(abort);
+            int p = ci;
+            if (!abort && (p < j || p >= M)) {
+                if (lane == 0) atomicCAS((int *)status, 0, (int)LA_ERR_TIMEOUT);
+                p = j;
+            }
+            // 2. fetch the winner row (and old row j if this workgroup owns p)
+            if (!abort) {
+                const int wp = p / R;
+                const bool need_d = (p != j) && (p >= rowbase && p < rowbase + nrows);
+                const uint64_t *wsrc = cand + ((size_t)buf * PANEL_MAXP + wp) * CAND_WORDS + 2;
+                const uint64_t *dsrc = diagw + (size_t)buf * W * NB_IN;
+                for (int wi = lane; wi < W * NB * (need_d ? 2 : 1); wi += 64) {
+                    const bool isd = wi >= W * NB;
+                    const int k = isd ? wi - W * NB : wi;
+                    const uint64_t *src = (isd ? dsrc : wsrc) + k;
+                    const uint64_t t0 = wall_clock64();
+                    uint64_t v = 0;
+                    for (uint32_t spin = 0;; ++spin) {
+                        v = ld_dev(src);
+                        if ((uint32_t)(v >> 32) == gs) break;
+                        if ((spin & 31) == 31 && ld_status(status) != 0) break;
+                        if (wall_clock64() - t0 > spin_ticks) {
+                            atomicCAS((int *)status, 0, (int)LA_ERR_TIMEOUT);
+                            break;
+                        }
+                    }
+                    (isd ? s_drow : s_urow[j & 1])[k] = (uint32_t)v;
                 }
-                __builtin_amdgcn_s_sleep(1);
+            }
+            if (lane == 0) {
+                s_abort = abort;
+                s_p = p;
+                s_piv[j] = p;
+                if (w == 0) ipiv[kglob + j] = (int64_t)kglob + p;
+            }
+        } else if (pend_j >= 0) {
+            // 1'. meanwhile waves 1-3 finish step pend_j's rank-1 update:
+            //     columns pend_j+2 .. NB-1 of every row below pend_j except
+            //     the rows the critical wave already brought up to date
+            const int jj = pend_j;
+            const T *uprev = reinterpret_cast<const T *>(s_urow[jj & 1]);
+            const int sk0 = s_skip[0], sk1 = s_skip[1];
+            const int r_first = max(0, jj + 1 - rowbase);
+            // each wave takes a contiguous third of the rows; loads are
+            // issued eight rows at a time ahead of the stores so the LDS
+            // latency is paid once per batch, not once per row
+            const int nr = nrows - r_first;
+            const int per = (nr + 2) / 3;
+            const int rb0 = r_first + (wave - 1) * per;
+            const int rb1 = min(nrows, rb0 + per);
+            for (int cc = jj + 2 + lane; cc < NB; cc += 64) {
+                const T u = uprev[cc];
+                int r = rb0;
+                for (; r + 8 <= rb1; r += 8) {
+                    T l[8], a[8];
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) {
+                        l[k] = blk[(r + k) * LDB + jj];
+                        a[k] = blk[(r + k) * LDB + cc];
+                    }
+#pragma unroll
+                    for (int k = 0; k < 8; ++k)
+                        if (r + k != sk0 && r + k != sk1) blk[(r + k) * LDB + cc] = a[k] - l[k] * u;
+                }
+                for (; r < rb1; ++r)
+                    if (r != sk0 && r != sk1) blk[r * LDB + cc] -= blk[r * LDB + jj] * u;
             }
         }
         __syncthreads();
+        PHASE_MARK(0);
         if (s_abort) return;
-        wg_argmax(ck, ci, s_key, s_idx);
-        int p = ci;
-        if (p < j || p >= M) {
-            if (tid == 0) atomicCAS((int *)status, 0, (int)LA_ERR_TIMEOUT);
-            p = j;
-        }
-        const int wp = p / R;
-
-        // 2. fetch the winner row (and old row j for the owner of p)
-        const bool need_d = (p != j) && (p >= rowbase && p < rowbase + nrows);
-        if (tid < W * NB || (need_d && tid >= 128 && tid - 128 < W * NB)) {
-            const bool isd = tid >= 128;
-            const int widx = isd ? tid - 128 : tid;
-            const uint64_t *src = isd ? diagw + (size_t)buf * W * NB_IN + widx
-                                      : cand + ((size_t)buf * PANEL_MAXP + wp) * CAND_WORDS + 2 + widx;
-            const uint64_t t0 = wall_clock64();
-            uint64_t v;
-            while (true) {
-                v = ld_dev(src);
-                if ((uint32_t)(v >> 32) == gs) break;
-                if (ld_status(status) != 0) break;
-                if (wall_clock64() - t0 > spin_ticks) {
-                    atomicCAS((int *)status, 0, (int)LA_ERR_TIMEOUT);
-                    break;
-                }
-            }
-            (isd ? s_drow : s_urow)[widx] = (uint32_t)v;
-        }
-        if (tid == 0) {
-            s_piv[j] = p;
-            if (w == 0) ipiv[kglob + j] = (int64_t)kglob + p;
-        }
-        __syncthreads();
+        const int p = s_p;
+        const bool own_j = (j >= rowbase && j < rowbase + nrows);
+        const bool own_p = (p != j) && (p >= rowbase && p < rowbase + nrows);
         // 3. swap rows j and p inside the panel
         if (p != j) {
-            if (j >= rowbase && j < rowbase + nrows) {
+            if (own_j) {
                 const int rl = j - rowbase;
                 for (int c = tid; c < NB; c += blockDim.x) blk[rl * LDB + c] = urow[c];
             }
-            if (need_d) {
+            if (own_p) {
                 const int rl = p - rowbase;
                 for (int c = tid; c < NB; c += blockDim.x) blk[rl * LDB + c] = drow[c];
             }
         }
         __syncthreads();
+        PHASE_MARK(2);
 
-        // 4. critical path (wave 0): multipliers, column j+1, next candidate
+        // 4. critical path (wave 0): multipliers, column j+1, next candidate.
+        //    This panel never runs for matrices small enough to be factored
+        //    in one piece, so the multipliers use one reciprocal per column.
         const T piv = urow[j];
         const int j1 = j + 1;
         if (wave == 0) {
+            const T rpiv = (piv != T(0)) ? T(1) / piv : T(1);
+            const T u1 = (j1 < NB) ? urow[j1] : T(0);
+            T lv[RPL], av[RPL];
+#pragma unroll
+            for (int q = 0; q < RPL; ++q) {
+                const int r = min(lane + 64 * q, nrows - 1);
+                lv[q] = blk[r * LDB + j];
+                av[q] = (j1 < NB) ? blk[r * LDB + j1] : T(0);
+            }
 #pragma unroll
             for (int q = 0; q < RPL; ++q) {
                 const int r = lane + 64 * q;
                 if (r < nrows && rowbase + r > j) {
-                    T l = blk[r * LDB + j];
-                    if (piv != T(0)) l = l / piv;
+                    const T l = lv[q] * rpiv;
                     blk[r * LDB + j] = l;
-                    if (j1 < NB) blk[r * LDB + j1] -= l * urow[j1];
+                    if (j1 < NB) blk[r * LDB + j1] = av[q] - l * u1;
                 }
             }
             // the next reads cross lanes: keep the compiler from moving them
@@ -364,19 +472,8 @@ void k_getrf_panel(T *A, int64_t lda, int M, int NB, int R, int kglob, int64_t *
             }
         }
         __syncthreads();
-        // 5. rest of the rank-1 update, overlapping the step-(j+1) exchange
-        if (j1 + 1 < NB) {
-            const int sk0 = s_skip[0], sk1 = s_skip[1];
-            const int r_first = max(0, j + 1 - rowbase);
-            const int ncol = NB - j1 - 1;
-            for (int e = tid; e < (nrows - r_first) * ncol; e += blockDim.x) {
-                const int r = r_first + e / ncol;
-                const int c = j1 + 1 + e % ncol;
-                if (r == sk0 || r == sk1) continue;
-                blk[r * LDB + c] -= blk[r * LDB + j] * urow[c];
-            }
-        }
-        __syncthreads();
+        PHASE_MARK(3);
+        pend_j = (j1 + 1 < NB) ? j : -1;
     }
 
     for (int e = tid; e < nrows * NB; e += blockDim.x) {
@@ -385,6 +482,11 @@ void k_getrf_panel(T *A, int64_t lda, int M, int NB, int R, int kglob, int64_t *
     }
     __syncthreads();
     if (w == 0 && tid < 64) panel_net_moves(NB, kglob, s_piv, swapmap);
+    if (w == 0) {
+        // L11 (rows 0..NB-1 of the panel, all owned here since R >= NB)
+        __shared__ T Xs[64][65];
+        invert_unit_lower<T>(blk, LDB, NB, Xs, linv);
+    }
 }
 
 // ------------------------------------------------------ single-WG panel
@@ -399,7 +501,7 @@ void k_getrf_panel(T *A, int64_t lda, int M, int NB, int R, int kglob, int64_t *
 template <typename T>
 __global__ __launch_bounds__(256)
 void k_getrf_panel_local(T *A, int64_t lda, int M, int NB, int kglob, int64_t *ipiv,
-                         int64_t *swapmap)
+                         int64_t *swapmap, T *linv)
 {
 #pragma clang fp contract(off)
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -477,6 +579,8 @@ void k_getrf_panel_local(T *A, int64_t lda, int M, int NB, int kglob, int64_t *i
     }
     __syncthreads();
     if (tid < 64) panel_net_moves(NB, kglob, s_piv, swapmap);
+    __shared__ T Xs[64][65];
+    invert_unit_lower<T>(blk, LDB, NB, Xs, linv);
 }
 
 // ------------------------------------------------------------ row swaps
@@ -535,34 +639,32 @@ void k_apply_rowmoves(T *A, int64_t lda, int n, int c_lo, int c_hi,
 }
 
 // ------------------------------------------------------------ unit TRSM
-// B (nb x ncols, in place, row-major) <- L^-1 B with L the nb x nb unit lower
-// triangle at Lm (ld ldl), nb <= 64.  A workgroup stages 64 columns of B and
-// the triangle in LDS and sweeps the columns of L: after step t, row t of
-// the block is final and rows below it have had row t's contribution
-// removed.
+// B (nb x ncols, in place, row-major) <- L^-1 B, given Linv = L^-1 (NB_IN x
+// NB_IN, produced by the panel kernel).  A workgroup stages 64 columns of B
+// and Linv in LDS; thread (column c, row group q) forms rows q, q+4, ...
+// No per-row dependency remains, so there is a single barrier.
 template <typename T>
 __global__ __launch_bounds__(256)
-void k_trsm_unit_lower(const T *Lm, int64_t ldl, int nb, T *B, int64_t ldb, int ncols)
+void k_trsm_unit_lower(const T *Linv, int nb, T *B, int64_t ldb, int ncols)
 {
-    __shared__ T Ls[64][65];
+    __shared__ T Li[64][65];
     __shared__ T Bs[64][65];
     const int tid = threadIdx.x;
     const int c0 = blockIdx.x * 64;
     const int cl = tid & 63, rq = tid >> 6;
     for (int e = tid; e < 64 * 64; e += 256) {
-        int i = e >> 6, t = e & 63;
-        Ls[i][t] = (i < nb && t < i) ? Lm[(int64_t)i * ldl + t] : T(0);
+        const int i = e >> 6, t = e & 63;
+        Li[i][t] = (i < nb && t < nb) ? Linv[i * NB_IN + t] : T(0);
     }
     for (int i = rq; i < 64; i += 4)
         Bs[i][cl] = (i < nb && c0 + cl < ncols) ? B[(int64_t)i * ldb + c0 + cl] : T(0);
     __syncthreads();
-    for (int t = 0; t < nb - 1; ++t) {
-        const T xt = Bs[t][cl];
-        for (int i = t + 1 + ((rq - (t + 1)) & 3); i < nb; i += 4) Bs[i][cl] -= Ls[i][t] * xt;
-        __syncthreads();
+    if (c0 + cl >= ncols) return;
+    for (int i = rq; i < nb; i += 4) {
+        T s = Bs[i][cl];                 // unit diagonal
+        for (int t = 0; t < i; ++t) s += Li[i][t] * Bs[t][cl];
+        B[(int64_t)i * ldb + c0 + cl] = s;
     }
-    if (c0 + cl < ncols)
-        for (int i = rq; i < nb; i += 4) B[(int64_t)i * ldb + c0 + cl] = Bs[i][cl];
 }
 
 // sign = parity of the number of real swaps in ipiv (lu.rs:140-152 flips the
@@ -595,48 +697,46 @@ constexpr size_t PANEL_LDS_MAX = 150 * 1024;
 
 template <typename T>
 static int launch_panel(la_ctx *ctx, T *Apanel, int64_t lda, int M, int NB, int kglob,
-                        int64_t *d_ipiv, const PanelWS &pw, int32_t *d_status, uint64_t ticks)
+                        int64_t *d_ipiv, const PanelWS &pw, int32_t *d_status, uint64_t ticks,
+                        T *linv)
 {
     // algorithmic traffic: the panel is read and written once
     ProfScope ps(ctx, LA_K_PANEL, (double)M * NB * NB, 2.0 * M * NB * sizeof(T));
-    static bool attr_local[2] = {false, false};
+    static size_t local_max[2] = {0, 0};
     const int ai0 = sizeof(T) == 8 ? 0 : 1;
+    if (!local_max[ai0]) local_max[ai0] = allow_dynamic_lds((const void *)k_getrf_panel_local<T>);
     const size_t local_bytes = (size_t)M * (NB + 1) * sizeof(T);
-    if (local_bytes <= 136 * 1024) {
-        if (!attr_local[ai0]) {
-            hipFuncSetAttribute((const void *)k_getrf_panel_local<T>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)PANEL_LDS_MAX);
-            attr_local[ai0] = true;
-        }
+    if (local_bytes <= std::min<size_t>(96 * 1024, local_max[ai0])) {
         hipLaunchKernelGGL(k_getrf_panel_local<T>, dim3(1), dim3(256), local_bytes, ctx->stream,
-                           Apanel, lda, M, NB, kglob, d_ipiv, pw.swapmap);
+                           Apanel, lda, M, NB, kglob, d_ipiv, pw.swapmap, linv);
         LA_HIP_CHECK(ctx, hipGetLastError());
         return LA_OK;
     }
-    int maxp = std::min(PANEL_MAXP, ctx->num_cus > 0 ? ctx->num_cus : 256);
-    // rows per workgroup: 64 per lane-row of the first wavefront
-    int rpl = (int)ceil_div(ceil_div(M, maxp), 64);
-    if (rpl < 1) rpl = 1;
-    if (rpl > 4)
-        return set_error(ctx, LA_ERR_ARG, "getrf panel: more than 65536 rows per panel");
+    // Workgroups poll every peer's slot once per column with device-coherent
+    // loads, so the poll traffic grows with P^2: hold P at 64 while the
+    // rows fit (64 rows per lane-row of the first wavefront, up to 4).
+    const int maxp = std::min(PANEL_MAXP, ctx->num_cus > 0 ? ctx->num_cus : 256);
+    // LDS: R x (NB+1) rows + 35 KB static (inverse workspace) must fit 160 KB
+    const int rpl_max = sizeof(T) == 8 ? 3 : 4;
+    int rpl = (int)std::min<int64_t>(rpl_max, std::max<int64_t>(1, ceil_div(ceil_div(M, 64), 64)));
     int R = 64 * rpl;
     int P = (int)ceil_div(M, R);
+    if (P > maxp) return set_error(ctx, LA_ERR_ARG, "getrf panel: too many rows");
     size_t shmem = (size_t)R * (NB + 1) * sizeof(T);
-    static bool attr_set[2][4] = {{false, false, false, false}, {false, false, false, false}};
+    static size_t coop_max[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
     const int ai = sizeof(T) == 8 ? 0 : 1;
     const void *fn = rpl == 1 ? (const void *)k_getrf_panel<T, 1>
                    : rpl == 2 ? (const void *)k_getrf_panel<T, 2>
                    : rpl == 3 ? (const void *)k_getrf_panel<T, 3>
                               : (const void *)k_getrf_panel<T, 4>;
-    if (!attr_set[ai][rpl - 1]) {
-        hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)PANEL_LDS_MAX);
-        attr_set[ai][rpl - 1] = true;
-    }
+    if (!coop_max[ai][rpl - 1]) coop_max[ai][rpl - 1] = allow_dynamic_lds(fn);
+    if (shmem > coop_max[ai][rpl - 1])
+        return set_error(ctx, LA_ERR_ARG, "getrf panel: rows per workgroup exceed LDS");
     uint64_t *candp = pw.slots;
     uint64_t *diagp = reinterpret_cast<uint64_t *>(pw.diagbuf);
     int64_t *swapmap = pw.swapmap;
     void *args[] = {&Apanel, &lda, &M, &NB, &R, &kglob, &d_ipiv, &candp, &diagp,
-                    &swapmap, &d_status, &ticks};
+                    &swapmap, &d_status, &ticks, &linv};
     LA_HIP_CHECK(ctx, hipLaunchCooperativeKernel(fn, dim3(P), dim3(256), args, (unsigned)shmem,
                                                  ctx->stream));
     return LA_OK;
@@ -658,14 +758,13 @@ static int launch_rowmoves(la_ctx *ctx, T *A, int64_t lda, int64_t n, int64_t c_
 }
 
 template <typename T>
-static int launch_trsm(la_ctx *ctx, const T *L, int64_t ldl, int nb, T *B, int64_t ldb,
-                       int64_t ncols)
+static int launch_trsm(la_ctx *ctx, const T *Linv, int nb, T *B, int64_t ldb, int64_t ncols)
 {
     if (ncols <= 0) return LA_OK;
     ProfScope ps(ctx, LA_K_TRSM, (double)nb * nb * ncols,
-                 (double)sizeof(T) * (2.0 * nb * ncols + 0.5 * nb * nb));
+                 (double)sizeof(T) * (2.0 * nb * ncols + (double)nb * nb));
     hipLaunchKernelGGL(k_trsm_unit_lower<T>, dim3((unsigned)ceil_div(ncols, 64)), dim3(256), 0,
-                       ctx->stream, L, ldl, nb, B, ldb, (int)ncols);
+                       ctx->stream, Linv, nb, B, ldb, (int)ncols);
     LA_HIP_CHECK(ctx, hipGetLastError());
     return LA_OK;
 }
@@ -684,8 +783,10 @@ int getrf_core(la_ctx *ctx, int64_t m, int64_t n, T *A, int64_t lda, int64_t *d_
     size_t row_bytes = 256;
     size_t diag_bytes = (size_t)2 * 2 * NB_IN * sizeof(uint64_t);
     size_t map_bytes = (size_t)(1 + 4 * NB_IN) * sizeof(int64_t);
+    constexpr int NLINV = NB_OUT / NB_IN;
+    size_t linv_bytes = (size_t)NLINV * NB_IN * NB_IN * sizeof(T);
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    size_t total = al(slot_bytes) + al(row_bytes) + al(diag_bytes) + al(map_bytes);
+    size_t total = al(slot_bytes) + al(row_bytes) + al(diag_bytes) + al(map_bytes) + al(linv_bytes);
     void *base = nullptr;
     int st = ws_get(ctx, WS_PANEL, total, &base);
     if (st) return st;
@@ -694,7 +795,8 @@ int getrf_core(la_ctx *ctx, int64_t m, int64_t n, T *A, int64_t lda, int64_t *d_
     pw.slots = (uint64_t *)b; b += al(slot_bytes);
     pw.rowbuf = b; b += al(row_bytes);
     pw.diagbuf = b; b += al(diag_bytes);
-    pw.swapmap = (int64_t *)b;
+    pw.swapmap = (int64_t *)b; b += al(map_bytes);
+    T *linv_ring = (T *)b;
 
     // reset slots: step tag 0xffffffff never occurs
     // step tag 0xffffffff never occurs: no stale word can validate
@@ -711,14 +813,15 @@ int getrf_core(la_ctx *ctx, int64_t m, int64_t n, T *A, int64_t lda, int64_t *d_
         for (int64_t k = K0; k < K0 + nbo; k += NB_IN) {
             const int nbi = (int)std::min<int64_t>(NB_IN, K0 + nbo - k);
             T *Ap = A + k * lda + k;
+            T *linv = linv_ring + (size_t)((k - K0) / NB_IN) * NB_IN * NB_IN;
             if ((st = launch_panel<T>(ctx, Ap, lda, (int)(m - k), nbi, (int)k, d_ipiv, pw,
-                                      d_status, ticks)))
+                                      d_status, ticks, linv)))
                 return st;
             if ((st = launch_rowmoves<T>(ctx, A, lda, n, k, k + nbi, pw.swapmap, d_perm)))
                 return st;
             const int64_t rest = K0 + nbo - (k + nbi);
             if (rest > 0) {
-                if ((st = launch_trsm<T>(ctx, Ap, lda, nbi, A + k * lda + k + nbi, lda, rest)))
+                if ((st = launch_trsm<T>(ctx, linv, nbi, A + k * lda + k + nbi, lda, rest)))
                     return st;
                 if (m - k - nbi > 0)
                     if ((st = gemm_rm<T>(ctx, m - k - nbi, rest, nbi, T(-1),
@@ -733,8 +836,8 @@ int getrf_core(la_ctx *ctx, int64_t m, int64_t n, T *A, int64_t lda, int64_t *d_
             T *B12 = A + K0 * lda + K0 + nbo;
             for (int64_t kb = K0; kb < K0 + nbo; kb += NB_IN) {
                 const int nbi = (int)std::min<int64_t>(NB_IN, K0 + nbo - kb);
-                if ((st = launch_trsm<T>(ctx, A + kb * lda + kb, lda, nbi,
-                                         A + kb * lda + K0 + nbo, lda, ncol)))
+                const T *linv_kb = linv_ring + (size_t)((kb - K0) / NB_IN) * NB_IN * NB_IN;
+                if ((st = launch_trsm<T>(ctx, linv_kb, nbi, A + kb * lda + K0 + nbo, lda, ncol)))
                     return st;
                 const int64_t below = K0 + nbo - (kb + nbi);
                 if (below > 0)
@@ -766,3 +869,19 @@ template int getrf_core<float>(la_ctx *, int64_t, int64_t, float *, int64_t, int
                                uint64_t *, int32_t *);
 
 }  // namespace lahip
+
+#ifdef LAHIP_PANEL_TIMING
+// Probe-only export of the panel phase clock (and reset).
+extern "C" int la_debug_panel_phase(unsigned long long *out8, int reset)
+{
+    if (hipMemcpyFromSymbol(out8, HIP_SYMBOL(lahip::g_panel_phase), 8 * sizeof(unsigned long long)) !=
+        hipSuccess)
+        return LA_ERR_HIP;
+    if (reset) {
+        unsigned long long z[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (hipMemcpyToSymbol(HIP_SYMBOL(lahip::g_panel_phase), z, sizeof z) != hipSuccess)
+            return LA_ERR_HIP;
+    }
+    return LA_OK;
+}
+#endif

@@ -230,8 +230,13 @@ extern "C" int la_dgesv_ir_dev(la_ctx *ctx, int64_t n, int64_t nrhs, const doubl
     }
     LA_HIP_CHECK(ctx, hipEventRecord(ev.e[2], ctx->stream));
     LA_HIP_CHECK(ctx, hipEventSynchronize(ev.e[2]));
-    int32_t fst = 0;
+    int32_t fst = 0, derr = 0;
     LA_HIP_CHECK(ctx, hipMemcpy(&fst, w.status, 4, hipMemcpyDeviceToHost));
+    LA_HIP_CHECK(ctx, hipMemcpy(&derr, ctx->d_scratch + SCR_ERR, 4, hipMemcpyDeviceToHost));
+    if (derr != 0) {
+        fill_i32(ctx, ctx->d_scratch + SCR_ERR, 1, 0);
+        if (fst == LA_OK) fst = derr;
+    }
     report->iterations = it;
     report->backward_error = eta;
     report->factor_ms = ev.ms(0, 1);

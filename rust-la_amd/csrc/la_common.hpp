@@ -34,6 +34,14 @@ __device__ inline T ld_dev(const T *p) {
 // wait until every store issued by this thread has completed
 __device__ inline void stores_done() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
+// SplitMix64 finaliser over a counter: the seeded generators' bit source.
+__device__ __host__ inline uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+constexpr uint64_t GOLDEN = 0x9E3779B97F4A7C15ULL;
+
 // Context: one device, one stream, a grow-only workspace, small device scratch
 // for status words.
 struct Workspace {
@@ -68,6 +76,9 @@ struct la_ctx {
     int32_t *h_pinned = nullptr;    // 4 KiB pinned host mirror
     std::string last_error;
     // kernel timing harness
+    uint32_t trsv_epoch = 0;        // tag of the triangular-solve exchange
+    int chol_nb[4] = {16, 16, 32, 32};   // batched Cholesky panel width per size bucket
+    bool chol_tuned = false;
     bool prof = false;
     std::vector<hipEvent_t> ev_pool;
     std::vector<lahip::ProfRecord> prof_pending;
@@ -111,6 +122,9 @@ struct ProfScope {
 };
 
 // ---- helpers implemented in util.hip --------------------------------------
+// Allow a kernel the dynamic LDS left over by its static LDS (160 KiB per CU
+// on gfx950); returns that limit in bytes and leaves no sticky HIP error.
+size_t allow_dynamic_lds(const void *kernel);
 uint64_t spin_limit_ticks(la_ctx *ctx);
 int fill_i32(la_ctx *ctx, int32_t *p, int64_t n, int32_t v);
 int fill_u64(la_ctx *ctx, uint64_t *p, int64_t n, uint64_t v);

@@ -21,22 +21,65 @@ enum { MODE_LU_FWD = 0, MODE_LU_BWD = 1, MODE_CH_FWD = 2, MODE_CH_BWD = 3 };
 constexpr int TB = 64;      // rows per block
 constexpr int NR = 16;      // right-hand sides per launch
 
+// Inverse of the rb x rb diagonal block held in Ts (LDS, FP64), written to
+// Xi (LDS).  Lower modes sweep rows downwards, upper modes upwards; for the
+// transposed Cholesky mode the upper triangle is L^T (U[i][t] = L[t][i]).
+// Four threads share each column's dot product.
+template <int MODE>
+__device__ void invert_diag_block(const double (*Ts)[TB + 1], int rb, double (*Xi)[TB + 1],
+                                  double (*part)[TB])
+{
+    constexpr bool LOWER = (MODE == MODE_LU_FWD || MODE == MODE_CH_FWD);
+    constexpr bool TRANS = (MODE == MODE_CH_BWD);
+    constexpr bool UNIT = (MODE == MODE_LU_FWD);
+    const int tid = threadIdx.x, c = tid & 63, q = tid >> 6;
+    for (int e = tid; e < TB * TB; e += blockDim.x) Xi[e / TB][e % TB] = 0.0;
+    __syncthreads();
+    for (int step = 0; step < rb; ++step) {
+        const int i = LOWER ? step : rb - 1 - step;
+        double s = 0.0;
+        if (LOWER) {
+            if (c < i)
+                for (int t = c + q; t < i; t += 4) s += Ts[i][t] * Xi[t][c];
+        } else {
+            if (c > i && c < rb)
+                for (int t = i + 1 + q; t <= c; t += 4)
+                    s += (TRANS ? Ts[t][i] : Ts[i][t]) * Xi[t][c];
+        }
+        part[q][c] = s;
+        __syncthreads();
+        if (q == 0 && c < rb) {
+            const double d = UNIT ? 1.0 : Ts[i][i];
+            const double tot = (part[0][c] + part[1][c]) + (part[2][c] + part[3][c]);
+            if (c == i) Xi[i][c] = 1.0 / d;
+            else if (LOWER ? (c < i) : (c > i)) Xi[i][c] = -tot / d;
+        }
+        __syncthreads();
+    }
+}
+
 template <typename TF, int MODE>
 __global__ __launch_bounds__(256)
 void k_trsv(int n, const TF *__restrict__ Tm, int64_t ldt, int nrhs, double *X, int64_t ldx,
-            int32_t *flags, int32_t *ticket, int32_t *err, uint64_t spin_ticks)
+            uint64_t *xw, int32_t *ticket, uint32_t epoch, int32_t *err, uint64_t spin_ticks)
 {
-    // No contraction: within a block the sweeps below apply the reference's
-    // update order (lu.rs:257-275, cholesky.rs:124-131) with a rounded
-    // product and a rounded subtraction per term.
+    // Single-block solves (n <= 64) keep the reference's update order with a
+    // rounded product and a rounded subtraction per term (lu.rs:257-275,
+    // cholesky.rs:124-131); larger solves multiply by inverted diagonal blocks.
 #pragma clang fp contract(off)
     constexpr bool FWD = (MODE == MODE_LU_FWD || MODE == MODE_CH_FWD);
     constexpr bool TRANS = (MODE == MODE_CH_BWD);
     constexpr bool UNIT = (MODE == MODE_LU_FWD);
-    __shared__ double xs[TB][NR];
-    __shared__ double red[4][TB][NR + 1];
-    __shared__ double Ts[TB][TB + 1];
+    // scratch: the diagonal block (during inversion), then the x slices of
+    // finished blocks, then the partial-sum reduction
+    __shared__ __align__(16) double scratch[TB * (TB + 1)];
+    __shared__ double Xi[TB][TB + 1];
+    __shared__ double part[4][TB];
     __shared__ int s_blk;
+    double (*Ts)[TB + 1] = reinterpret_cast<double (*)[TB + 1]>(scratch);
+    uint32_t *xs32 = reinterpret_cast<uint32_t *>(scratch);
+    const double *xs = scratch;                              // [chunk][64][nrhs]
+    double (*red)[TB][NR] = reinterpret_cast<double (*)[TB][NR]>(scratch);  // [4][64][NR]
 
     const int tid = threadIdx.x;
     const int nblocks = (n + TB - 1) / TB;
@@ -46,88 +89,163 @@ void k_trsv(int n, const TF *__restrict__ Tm, int64_t ldt, int nrhs, double *X, 
     const int b = FWD ? t : nblocks - 1 - t;
     const int r0 = b * TB;
     const int rb = min(TB, n - r0);
+    const bool single = (nblocks == 1);
+
+    // invert the diagonal block first: this overlaps the wait for the
+    // blocks this one depends on
+    if (!single) {
+        for (int e = tid; e < TB * TB; e += blockDim.x) {
+            const int a = e / TB, c = e - a * TB;
+            Ts[a][c] = (a < rb && c < rb) ? (double)Tm[(int64_t)(r0 + a) * ldt + r0 + c] : 0.0;
+        }
+        __syncthreads();
+        invert_diag_block<MODE>(Ts, rb, Xi, part);
+        __syncthreads();
+    }
 
     double acc[NR];
 #pragma unroll
     for (int r = 0; r < NR; ++r) acc[r] = 0.0;
-
-    // thread mapping for the off-diagonal products
-    //   non-transposed: row i = tid/4, 16-column quarter q = tid%4
-    //   transposed    : row i = tid%64 (coalesced along L's row), quarter q = tid/64
     const int i = TRANS ? (tid & 63) : (tid >> 2);
     const int q = TRANS ? (tid >> 6) : (tid & 3);
+    // Blocks per chunk: small, so that a workgroup never sits on a large
+    // batch of finished blocks while its immediate predecessor completes
+    // (the batch right before the last dependency is on the critical path).
+    const int CH = max(1, min(8, 64 / max(nrhs, 1)));
+    const int words_per_block = TB * nrhs * 2;
 
-    for (int step = 0; step < t; ++step) {
-        const int c = FWD ? step : nblocks - 1 - step;
-        const int c0 = c * TB;
-        const int cb = min(TB, n - c0);
-        if (tid == 0) {
-            const uint64_t t0 = wall_clock64();
-            while (__hip_atomic_load(flags + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
-                if (wall_clock64() - t0 > spin_ticks) {
-                    atomicCAS((int *)err, 0, (int)LA_ERR_TIMEOUT);
-                    break;
+    // fold in finished blocks: all but the last dependency in chunks of CH,
+    // then the last one alone (it is the one on the critical path)
+    int done = 0;
+    while (done < t) {
+        const int chunk = (done < t - 1) ? min(CH, t - 1 - done) : 1;
+        // gather the tagged x words of blocks done .. done+chunk-1: sixteen
+        // independent loads in flight per thread, then validate; only words
+        // that are not yet current are polled again
+        const int nwords = chunk * words_per_block;
+        for (int e0 = tid; e0 < nwords; e0 += 16 * 256) {
+            uint64_t v[16];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const int e = e0 + k * 256;
+                v[k] = 0;
+                if (e < nwords) {
+                    const int kb = e / words_per_block, wi = e - kb * words_per_block;
+                    const int step = done + kb;
+                    const int c = FWD ? step : nblocks - 1 - step;
+                    v[k] = ld_dev(xw + (size_t)c * words_per_block + wi);
                 }
-                __builtin_amdgcn_s_sleep(1);
             }
-        }
-        __syncthreads();
-        for (int e = tid; e < cb * nrhs; e += blockDim.x) {
-            int col = e / nrhs, r = e - col * nrhs;
-            xs[col][r] = ld_dev(X + (int64_t)(c0 + col) * ldx + r);
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const int e = e0 + k * 256;
+                if (e >= nwords) continue;
+                if ((uint32_t)(v[k] >> 32) != epoch) {
+                    const int kb = e / words_per_block, wi = e - kb * words_per_block;
+                    const int step = done + kb;
+                    const int c = FWD ? step : nblocks - 1 - step;
+                    const uint64_t *src = xw + (size_t)c * words_per_block + wi;
+                    const uint64_t t0 = wall_clock64();
+                    for (uint32_t spin = 0;; ++spin) {
+                        __builtin_amdgcn_s_sleep(1);
+                        v[k] = ld_dev(src);
+                        if ((uint32_t)(v[k] >> 32) == epoch) break;
+                        // a wait that failed anywhere ends every other wait quickly
+                        if ((spin & 63) == 63 && ld_dev(err) != 0) break;
+                        if (wall_clock64() - t0 > spin_ticks) {
+                            atomicCAS((int *)err, 0, (int)LA_ERR_TIMEOUT);
+                            break;
+                        }
+                    }
+                }
+                xs32[e] = (uint32_t)v[k];
+            }
         }
         __syncthreads();
         if (i < rb) {
+            for (int k = 0; k < chunk; ++k) {
+                const int step = done + k;
+                const int c = FWD ? step : nblocks - 1 - step;
+                const int c0 = c * TB;
+                const int cb = min(TB, n - c0);
+                const double *xk = xs + (size_t)k * TB * nrhs;
 #pragma unroll 4
-            for (int cc = 0; cc < 16; ++cc) {
-                const int col = q * 16 + cc;
-                if (col < cb) {
-                    double tv;
-                    if (!TRANS) tv = (double)Tm[(int64_t)(r0 + i) * ldt + c0 + col];
-                    else        tv = (double)Tm[(int64_t)(c0 + col) * ldt + r0 + i];
+                for (int cc = 0; cc < 16; ++cc) {
+                    const int col = q * 16 + cc;
+                    if (col < cb) {
+                        double tv;
+                        if (!TRANS) tv = (double)Tm[(int64_t)(r0 + i) * ldt + c0 + col];
+                        else        tv = (double)Tm[(int64_t)(c0 + col) * ldt + r0 + i];
 #pragma unroll
-                    for (int r = 0; r < NR; ++r)
-                        if (r < nrhs) acc[r] += tv * xs[col][r];
+                        for (int r = 0; r < NR; ++r)
+                            if (r < nrhs) acc[r] = __builtin_fma(tv, xk[col * nrhs + r], acc[r]);
+                    }
                 }
             }
         }
         __syncthreads();
+        done += chunk;
     }
 
-    // fold partial sums, form the right-hand side of the diagonal solve
+    // reduce partial sums over the four column groups
 #pragma unroll
     for (int r = 0; r < NR; ++r) red[q][i][r] = acc[r];
-    for (int e = tid; e < TB * TB; e += blockDim.x) {
-        const int a = e / TB, bcol = e - a * TB;
-        Ts[a][bcol] = (a < rb && bcol < rb) ? (double)Tm[(int64_t)(r0 + a) * ldt + r0 + bcol] : 0.0;
-    }
     __syncthreads();
 
-    // diagonal triangle: wave w solves right-hand sides w, w+4, ...; lane = row
     const int lane = tid & 63, wv = tid >> 6;
-    for (int r = wv; r < nrhs; r += 4) {
-        double y = 0.0;
-        if (lane < rb)
-            y = X[(int64_t)(r0 + lane) * ldx + r] -
-                ((red[0][lane][r] + red[1][lane][r]) + (red[2][lane][r] + red[3][lane][r]));
-        if (FWD) {
-            for (int s = 0; s < rb; ++s) {
-                if (!UNIT && lane == s) y = y / Ts[s][s];
-                const double ys = __shfl(y, s);
-                if (lane > s && lane < rb) y -= Ts[lane][s] * ys;
-            }
-        } else {
-            for (int s = rb - 1; s >= 0; --s) {
-                if (lane == s) y = y / Ts[s][s];
-                const double ys = __shfl(y, s);
-                if (lane < s) y -= (TRANS ? Ts[s][lane] : Ts[lane][s]) * ys;
-            }
+    if (single) {
+        // substitution in the reference's order (no contraction)
+        for (int e = tid; e < TB * TB; e += blockDim.x) {
+            const int a = e / TB, c = e - a * TB;
+            Xi[a][c] = (a < rb && c < rb) ? (double)Tm[(int64_t)(r0 + a) * ldt + r0 + c] : 0.0;
         }
-        if (lane < rb) st_dev(X + (int64_t)(r0 + lane) * ldx + r, y);
+        __syncthreads();
+        for (int r = wv; r < nrhs; r += 4) {
+            double y = (lane < rb) ? X[(int64_t)(r0 + lane) * ldx + r] : 0.0;
+            if (FWD) {
+                for (int s = 0; s < rb; ++s) {
+                    if (!UNIT && lane == s) y = y / Xi[s][s];
+                    const double ys = __shfl(y, s);
+                    if (lane > s && lane < rb) y -= Xi[lane][s] * ys;
+                }
+            } else {
+                for (int s = rb - 1; s >= 0; --s) {
+                    if (lane == s) y = y / Xi[s][s];
+                    const double ys = __shfl(y, s);
+                    if (lane < s) y -= (TRANS ? Xi[s][lane] : Xi[lane][s]) * ys;
+                }
+            }
+            if (lane < rb) X[(int64_t)(r0 + lane) * ldx + r] = y;
+        }
+        return;
     }
-    stores_done();
+    // y = b - partial sums (into part-like scratch: reuse red[0])
+    __shared__ double ys[TB][NR];
+    for (int e = tid; e < TB * nrhs; e += blockDim.x) {
+        const int a = e / nrhs, r = e - a * nrhs;
+        double y = 0.0;
+        if (a < rb)
+            y = X[(int64_t)(r0 + a) * ldx + r] -
+                ((red[0][a][r] + red[1][a][r]) + (red[2][a][r] + red[3][a][r]));
+        ys[a][r] = y;
+    }
     __syncthreads();
-    if (tid == 0) st_dev(flags + b, 1);
+    // x = inv(T_bb) y, published as tagged words and stored to X
+    const uint64_t tag = (uint64_t)epoch << 32;
+    // every one of the 64 rows is published (padding rows as zero): readers
+    // fetch whole blocks
+    for (int e = tid; e < TB * nrhs; e += blockDim.x) {
+        const int a = e / nrhs, r = e - a * nrhs;
+        double s = 0.0;
+        if (a < rb) {
+            for (int c = 0; c < rb; ++c) s = __builtin_fma(Xi[a][c], ys[c][r], s);
+            X[(int64_t)(r0 + a) * ldx + r] = s;
+        }
+        const uint64_t bits = (uint64_t)__double_as_longlong(s);
+        uint64_t *dst = xw + (size_t)b * words_per_block + ((size_t)a * nrhs + r) * 2;
+        st_dev(dst + 0, tag | (bits & 0xffffffffull));
+        st_dev(dst + 1, tag | (bits >> 32));
+    }
 }
 
 __global__ void k_zero_i32(int32_t *p, int n) {
@@ -150,23 +268,33 @@ static int launch_trsv_mode(la_ctx *ctx, int64_t n, const TF *Tm, int64_t ldt, i
                             double *X, int64_t ldx)
 {
     const int nblocks = (int)ceil_div(n, TB);
-    // per-block flags followed by the ticket counter
+    // ticket word, then the tagged x words (2 per value): blocks x 64 x NR x 2
+    const size_t words = (size_t)nblocks * TB * NR * 2;
+    const size_t bytes = 256 + words * sizeof(uint64_t);
     void *p = nullptr;
-    int st = ws_get(ctx, WS_SYNC, ((size_t)nblocks + 1) * sizeof(int32_t), &p);
+    const size_t before = ctx->ws[WS_SYNC].size;
+    int st = ws_get(ctx, WS_SYNC, bytes, &p);
     if (st) return st;
-    int32_t *flags = (int32_t *)p;
-    int32_t *ticket = flags + nblocks;
-    int32_t *d_err = ctx->d_scratch;   // word 0: sticky device error
+    int32_t *ticket = (int32_t *)p;
+    uint64_t *xw = (uint64_t *)((char *)p + 256);
+    if (ctx->ws[WS_SYNC].size != before) {
+        // fresh buffer: tag 0 never matches a live epoch (epochs start at 1)
+        if ((st = fill_u64(ctx, (uint64_t *)p, (int64_t)(ctx->ws[WS_SYNC].size / 8), 0)))
+            return st;
+        ctx->trsv_epoch = 0;
+    }
+    int32_t *d_err = ctx->d_scratch + SCR_ERR;
     const uint64_t ticks = spin_limit_ticks(ctx);
     // algorithmic: one triangle of the factor + right-hand sides in and out
     ProfScope ps(ctx, LA_K_TRSV, (double)n * n * nrhs,
                  0.5 * n * n * sizeof(TF) + 2.0 * n * nrhs * 8.0);
-    for (int64_t r0 = 0; r0 < nrhs; r0 += NR) {
-        const int nr = (int)std::min<int64_t>(NR, nrhs - r0);
-        hipLaunchKernelGGL(k_zero_i32, dim3((unsigned)ceil_div(nblocks + 1, 256)), dim3(256), 0,
-                           ctx->stream, flags, nblocks + 1);
+    for (int64_t rr = 0; rr < nrhs; rr += NR) {
+        const int nr = (int)std::min<int64_t>(NR, nrhs - rr);
+        uint32_t epoch = ++ctx->trsv_epoch;
+        if (epoch == 0) epoch = ++ctx->trsv_epoch;
+        if ((st = fill_i32(ctx, ticket, 1, 0))) return st;
         hipLaunchKernelGGL((k_trsv<TF, MODE>), dim3(nblocks), dim3(256), 0, ctx->stream,
-                           (int)n, Tm, ldt, nr, X + r0, ldx, flags, ticket, d_err, ticks);
+                           (int)n, Tm, ldt, nr, X + rr, ldx, xw, ticket, epoch, d_err, ticks);
         LA_HIP_CHECK(ctx, hipGetLastError());
     }
     return LA_OK;

@@ -37,6 +37,19 @@ __global__ void k_util_diag_zero(int64_t n, const double *LU, int64_t ld, int32_
 
 static inline unsigned nblk(int64_t n, int b) { return (unsigned)ceil_div(n, b); }
 
+size_t allow_dynamic_lds(const void *kernel) {
+    hipFuncAttributes attr;
+    size_t stat = 0;
+    if (hipFuncGetAttributes(&attr, kernel) == hipSuccess) stat = attr.sharedSizeBytes;
+    const size_t total = 160 * 1024;
+    size_t dyn = stat < total ? total - stat : 0;
+    if (hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn) !=
+        hipSuccess)
+        dyn = 64 * 1024 - stat;
+    (void)hipGetLastError();   // never leave a sticky error behind
+    return dyn;
+}
+
 uint64_t spin_limit_ticks(la_ctx *ctx) {
     int rate_khz = 0;
     if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, ctx->device) !=
@@ -81,12 +94,6 @@ int check_diag_zero(la_ctx *ctx, int64_t n, const double *LU, int64_t ld, int32_
 // ------------------------------------------------------------ generators
 // SplitMix64 finaliser over a counter; identical to the CPU generator the
 // tests use, so uniform draws are bit-identical on both sides.
-__device__ __host__ inline uint64_t mix64(uint64_t z) {
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
-    return z ^ (z >> 31);
-}
-constexpr uint64_t GOLDEN = 0x9E3779B97F4A7C15ULL;
 
 __global__ void k_fill_uniform(uint64_t key, int64_t count, double *out) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

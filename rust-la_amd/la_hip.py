@@ -123,6 +123,7 @@ _OPTIONAL = {
     "la_dpotrf_potrs_batched_dev": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "la_dgesv_ir_batched_dev": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_double,
                                 ctypes.c_int, _vp, _vp, _vp],
+    "la_gen_spd_batch_dev": [_vp, _i64, _vp, _vp, ctypes.c_uint64, ctypes.c_uint64, _vp, _vp],
     "la_autotune": [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p],
     "la_autotune_load": [_vp, ctypes.c_char_p],
 }
@@ -331,6 +332,27 @@ class Context:
                                                    b_stream, _ptr(b), _ptr(xstar)),
                     "gen_svd_matrix")
 
+    # ---- batched uneven solvers ------------------------------------------
+    def batch_plan(self, kind, sizes, nrhs=None, prec=0):
+        return BatchPlan(self, kind, sizes, nrhs, prec)
+
+    def potrf_potrs_batched_dev(self, plan, offA, offB, A, L, B, X, status):
+        self._check(self.lib.la_dpotrf_potrs_batched_dev(self.h, plan.h, _ptr(offA), _ptr(offB),
+                                                         _ptr(A), _ptr(L), _ptr(B), _ptr(X),
+                                                         _ptr(status)), "dpotrf_potrs_batched")
+
+    def gen_spd_batch_dev(self, d_n, d_offA, seed, stream_base, A, G):
+        self._check(self.lib.la_gen_spd_batch_dev(self.h, d_n.numel(), _ptr(d_n), _ptr(d_offA),
+                                                  seed, stream_base, _ptr(A), _ptr(G)),
+                    "gen_spd_batch")
+
+    def autotune(self, kind, nmin, nmax, cache_path=None):
+        self._check(self.lib.la_autotune(self.h, kind, nmin, nmax,
+                                         cache_path.encode() if cache_path else None), "autotune")
+
+    def autotune_load(self, cache_path):
+        self._check(self.lib.la_autotune_load(self.h, cache_path.encode()), "autotune_load")
+
     def backward_error_dev(self, A, X, B):
         eta = ctypes.c_double()
         self._check(self.lib.la_backward_error_dev(self.h, A.shape[0], B.shape[1], _ptr(A),
@@ -338,3 +360,33 @@ class Context:
                                                    B.stride(0), ctypes.byref(eta)),
                     "backward_error")
         return eta.value
+
+
+class BatchPlan:
+    """la_batch_plan: sorted problem order, device queue, captured graph."""
+
+    def __init__(self, ctx, kind, sizes, nrhs=None, prec=0):
+        self.ctx = ctx
+        n = np.ascontiguousarray(sizes, dtype=np.int32)
+        r = np.ascontiguousarray(nrhs, dtype=np.int32) if nrhs is not None else None
+        h = _vp()
+        st = ctx.lib.la_batch_plan_create(ctx.h, kind, len(n), n.ctypes.data_as(_i32p),
+                                          r.ctypes.data_as(_i32p) if r is not None else None,
+                                          prec, ctypes.byref(h))
+        ctx._check(st, "batch_plan_create")
+        self.h = h
+        ctx.lib.la_batch_plan_describe.restype = ctypes.c_char_p
+
+    def describe(self):
+        return self.ctx.lib.la_batch_plan_describe(self.h).decode()
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.la_batch_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
