@@ -769,11 +769,24 @@ static int launch_trsm(la_ctx *ctx, const T *Linv, int nb, T *B, int64_t ldb, in
     return LA_OK;
 }
 
+// Trailing updates: native MFMA for T, or FP32 data with BF16/FP16-rounded
+// operands on the low-precision MFMA (upd_prec = LA_PREC_BF16 / LA_PREC_FP16).
+template <typename T>
+static int trail_gemm(la_ctx *ctx, int upd_prec, int64_t M, int64_t N, int64_t K, const T *A,
+                      int64_t lda, const T *B, int64_t ldb, T *C, int64_t ldc)
+{
+    if constexpr (sizeof(T) == 4) {
+        if (upd_prec == LA_PREC_BF16 || upd_prec == LA_PREC_FP16)
+            return gemm_lp(ctx, upd_prec, M, N, K, -1.f, A, lda, B, ldb, 1.f, C, ldc);
+    }
+    return gemm_rm<T>(ctx, M, N, K, T(-1), A, lda, B, ldb, T(1), C, ldc, false, false);
+}
+
 // In-place LU.  d_ipiv receives LAPACK-style swaps (global rows), d_perm the
 // permutation vector (may be null), d_status LA_OK / LA_ERR_TIMEOUT.
 template <typename T>
 int getrf_core(la_ctx *ctx, int64_t m, int64_t n, T *A, int64_t lda, int64_t *d_ipiv,
-               uint64_t *d_perm, int32_t *d_status)
+               uint64_t *d_perm, int32_t *d_status, int upd_prec)
 {
     if (m > INT32_MAX / 2 || n > INT32_MAX / 2)
         return set_error(ctx, LA_ERR_ARG, "getrf: extent too large");
@@ -824,10 +837,9 @@ int getrf_core(la_ctx *ctx, int64_t m, int64_t n, T *A, int64_t lda, int64_t *d_
                 if ((st = launch_trsm<T>(ctx, linv, nbi, A + k * lda + k + nbi, lda, rest)))
                     return st;
                 if (m - k - nbi > 0)
-                    if ((st = gemm_rm<T>(ctx, m - k - nbi, rest, nbi, T(-1),
-                                         A + (k + nbi) * lda + k, lda, A + k * lda + k + nbi,
-                                         lda, T(1), A + (k + nbi) * lda + k + nbi, lda, false,
-                                         false)))
+                    if ((st = trail_gemm<T>(ctx, upd_prec, m - k - nbi, rest, nbi,
+                                            A + (k + nbi) * lda + k, lda, A + k * lda + k + nbi,
+                                            lda, A + (k + nbi) * lda + k + nbi, lda)))
                         return st;
             }
         }
@@ -841,15 +853,16 @@ int getrf_core(la_ctx *ctx, int64_t m, int64_t n, T *A, int64_t lda, int64_t *d_
                     return st;
                 const int64_t below = K0 + nbo - (kb + nbi);
                 if (below > 0)
-                    if ((st = gemm_rm<T>(ctx, below, ncol, nbi, T(-1), A + (kb + nbi) * lda + kb,
-                                         lda, A + kb * lda + K0 + nbo, lda, T(1),
-                                         A + (kb + nbi) * lda + K0 + nbo, lda, false, false)))
+                    if ((st = trail_gemm<T>(ctx, upd_prec, below, ncol, nbi,
+                                            A + (kb + nbi) * lda + kb, lda,
+                                            A + kb * lda + K0 + nbo, lda,
+                                            A + (kb + nbi) * lda + K0 + nbo, lda)))
                         return st;
             }
             if (K0 + nbo < m)
-                if ((st = gemm_rm<T>(ctx, m - K0 - nbo, ncol, nbo, T(-1),
-                                     A + (K0 + nbo) * lda + K0, lda, B12, lda, T(1),
-                                     A + (K0 + nbo) * lda + K0 + nbo, lda, false, false)))
+                if ((st = trail_gemm<T>(ctx, upd_prec, m - K0 - nbo, ncol, nbo,
+                                        A + (K0 + nbo) * lda + K0, lda, B12, lda,
+                                        A + (K0 + nbo) * lda + K0 + nbo, lda)))
                     return st;
         }
     }
@@ -864,9 +877,9 @@ int pivsign(la_ctx *ctx, const int64_t *d_ipiv, int64_t kmax, int32_t *d_info)
 }
 
 template int getrf_core<double>(la_ctx *, int64_t, int64_t, double *, int64_t, int64_t *,
-                                uint64_t *, int32_t *);
+                                uint64_t *, int32_t *, int);
 template int getrf_core<float>(la_ctx *, int64_t, int64_t, float *, int64_t, int64_t *,
-                               uint64_t *, int32_t *);
+                               uint64_t *, int32_t *, int);
 
 }  // namespace lahip
 

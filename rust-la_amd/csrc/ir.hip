@@ -35,6 +35,47 @@ __global__ void k_convert_f32(int64_t rows, int64_t cols, const double *A, int64
     }
 }
 
+// Column maxima of |A| (non-negative doubles compared as bit patterns):
+// grid (column blocks of 256, row chunks of 256).
+__global__ __launch_bounds__(256) void k_colmax(int64_t n, const double *A, int64_t lda,
+                                                unsigned long long *colmax)
+{
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (j >= n) return;
+    const int64_t i0 = (int64_t)blockIdx.y * 256, i1 = min(n, i0 + 256);
+    double m = 0.0;
+    for (int64_t i = i0; i < i1; ++i) m = fmax(m, fabs(A[i * lda + j]));
+    atomicMax(colmax + j, (unsigned long long)__double_as_longlong(m));
+}
+
+// s_j = 2^-e_j with max_i |A_ij| = f * 2^e_j, f in [1, 2): powers of two,
+// so scaling is exact; A32 = fl32(A S).
+__global__ void k_colscale(int64_t n, const unsigned long long *colmax, double *s)
+{
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const double m = __longlong_as_double((long long)colmax[j]);
+    s[j] = (m > 0.0 && isfinite(m)) ? ldexp(1.0, -ilogb(m)) : 1.0;
+}
+
+__global__ void k_convert_f32_scaled(int64_t n, const double *A, int64_t lda, const double *s,
+                                     float *B, int64_t ldb)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * n) return;
+    const int64_t r = e / n, c = e - r * n;
+    B[r * ldb + c] = (float)(A[r * lda + c] * s[c]);
+}
+
+// x_j *= s_j for the solution rows (x = S y)
+__global__ void k_scale_rows_by(int64_t n, int nrhs, const double *s, double *X, int64_t ldx)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * nrhs) return;
+    const int64_t i = e / nrhs, r = e - i * nrhs;
+    X[i * ldx + r] *= s[i];
+}
+
 // Non-negative doubles order like their bit patterns, so a running maximum
 // can use the 64-bit integer atomicMax.
 __device__ inline void atomic_max_nonneg(double *addr, double v) {
@@ -128,6 +169,9 @@ int residual(la_ctx *ctx, int64_t n, int64_t nrhs, const double *A, int64_t lda,
 struct IrWork {
     float *A32 = nullptr;
     int64_t ld32 = 0;
+    double *colscale = nullptr;          // per-column 2^k scale (BF16/FP16 modes)
+    unsigned long long *colmax = nullptr;
+    int prec = LA_PREC_FP32;
     uint64_t *perm = nullptr;
     int64_t *ipiv = nullptr;
     double *R = nullptr, *D = nullptr, *norms = nullptr;
@@ -141,9 +185,9 @@ static int ir_alloc(la_ctx *ctx, int64_t n, int64_t nrhs, IrWork &w)
     size_t b32 = al((size_t)n * ld32 * sizeof(float));
     size_t bperm = al((size_t)n * 8), bipiv = al((size_t)n * 8);
     size_t brd = al((size_t)n * nrhs * 8);
-    size_t bn = al(8 * 8), bst = al(4 * 4);
+    size_t bn = al(8 * 8), bst = al(4 * 4), bsc = al((size_t)n * 8);
     void *base = nullptr;
-    int st = ws_get(ctx, WS_LP, b32 + bperm + bipiv + 2 * brd + bn + bst, &base);
+    int st = ws_get(ctx, WS_LP, b32 + bperm + bipiv + 2 * brd + bn + bst + 2 * bsc, &base);
     if (st) return st;
     char *p = (char *)base;
     w.A32 = (float *)p; p += b32;
@@ -153,7 +197,9 @@ static int ir_alloc(la_ctx *ctx, int64_t n, int64_t nrhs, IrWork &w)
     w.R = (double *)p; p += brd;
     w.D = (double *)p; p += brd;
     w.norms = (double *)p; p += bn;
-    w.status = (int32_t *)p;
+    w.status = (int32_t *)p; p += bst;
+    w.colscale = (double *)p; p += bsc;
+    w.colmax = (unsigned long long *)p;
     return LA_OK;
 }
 
@@ -165,7 +211,14 @@ static int lp_solve(la_ctx *ctx, int64_t n, int64_t nrhs, const IrWork &w, const
     if ((st = gather_rows(ctx, n, nrhs, w.perm, rhs, ldr, out, ldo))) return st;
     if ((st = trsv_blocked<float>(ctx, 'L', false, true, n, w.A32, w.ld32, nrhs, out, ldo)))
         return st;
-    return trsv_blocked<float>(ctx, 'U', false, false, n, w.A32, w.ld32, nrhs, out, ldo);
+    if ((st = trsv_blocked<float>(ctx, 'U', false, false, n, w.A32, w.ld32, nrhs, out, ldo)))
+        return st;
+    if (w.prec == LA_PREC_BF16 || w.prec == LA_PREC_FP16) {   // x = S y
+        hipLaunchKernelGGL(k_scale_rows_by, dim3((unsigned)ceil_div(n * nrhs, 256)), dim3(256), 0,
+                           ctx->stream, n, (int)nrhs, w.colscale, out, ldo);
+        LA_HIP_CHECK(ctx, hipGetLastError());
+    }
+    return LA_OK;
 }
 
 }  // namespace lahip
@@ -193,18 +246,32 @@ extern "C" int la_dgesv_ir_dev(la_ctx *ctx, int64_t n, int64_t nrhs, const doubl
     if (!ctx || !A || !B || !X || !report) return LA_ERR_ARG;
     if (n <= 0 || nrhs <= 0 || nrhs > 16 || lda < n || ldb < nrhs || ldx < nrhs)
         return LA_ERR_ARG;
-    if (prec != LA_PREC_FP32) return set_error(ctx, LA_ERR_ARG, "dgesv_ir: precision not supported");
+    if (prec != LA_PREC_FP32 && prec != LA_PREC_BF16 && prec != LA_PREC_FP16)
+        return set_error(ctx, LA_ERR_ARG, "dgesv_ir: precision not supported");
     std::memset(report, 0, sizeof(*report));
     report->prec = prec;
     IrWork w;
     int st = ir_alloc(ctx, n, nrhs, w);
     if (st) return st;
     EvTimer ev;
+    w.prec = prec;
     LA_HIP_CHECK(ctx, hipEventRecord(ev.e[0], ctx->stream));
-    hipLaunchKernelGGL(k_convert_f32, dim3((unsigned)ceil_div(n * n, 1024)), dim3(256), 0,
-                       ctx->stream, n, n, A, lda, w.A32, w.ld32);
+    if (prec == LA_PREC_FP32) {
+        hipLaunchKernelGGL(k_convert_f32, dim3((unsigned)ceil_div(n * n, 1024)), dim3(256), 0,
+                           ctx->stream, n, n, A, lda, w.A32, w.ld32);
+    } else {
+        // per-column power-of-two scaling keeps the rounded operands in range
+        if ((st = fill_u64(ctx, (uint64_t *)w.colmax, n, 0))) return st;
+        hipLaunchKernelGGL(k_colmax, dim3((unsigned)ceil_div(n, 256), (unsigned)ceil_div(n, 256)),
+                           dim3(256), 0, ctx->stream, n, A, lda, w.colmax);
+        hipLaunchKernelGGL(k_colscale, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0,
+                           ctx->stream, n, w.colmax, w.colscale);
+        hipLaunchKernelGGL(k_convert_f32_scaled, dim3((unsigned)ceil_div(n * n, 256)), dim3(256),
+                           0, ctx->stream, n, A, lda, w.colscale, w.A32, w.ld32);
+    }
     LA_HIP_CHECK(ctx, hipGetLastError());
-    if ((st = getrf_core<float>(ctx, n, n, w.A32, w.ld32, w.ipiv, w.perm, w.status))) return st;
+    if ((st = getrf_core<float>(ctx, n, n, w.A32, w.ld32, w.ipiv, w.perm, w.status, prec)))
+        return st;
     LA_HIP_CHECK(ctx, hipEventRecord(ev.e[1], ctx->stream));
     // singular low-precision factor: report and stop
     // (exact zero on U's diagonal makes every correction infinite)

@@ -147,9 +147,13 @@ int gemm_rm(la_ctx *ctx, int64_t M, int64_t N, int64_t K, T alpha,
 // d_ipiv (device, min(m,n)) receives LAPACK-style swaps (row k exchanged
 // with row d_ipiv[k], global 0-based); d_perm (device, m, may be null) the
 // permutation vector; d_status LA_OK or LA_ERR_TIMEOUT.
+// upd_prec: LA_PREC_FP32/FP64 = native MFMA trailing update; LA_PREC_BF16 /
+// LA_PREC_FP16 (T = float only) = operands rounded to that format.
 template <typename T>
 int getrf_core(la_ctx *ctx, int64_t m, int64_t n, T *A, int64_t lda, int64_t *d_ipiv,
-               uint64_t *d_perm, int32_t *d_status);
+               uint64_t *d_perm, int32_t *d_status, int upd_prec = LA_PREC_FP32);
+int gemm_lp(la_ctx *ctx, int prec, int64_t M, int64_t N, int64_t K, float alpha, const float *A,
+            int64_t lda, const float *B, int64_t ldb, float beta, float *C, int64_t ldc);
 // d_info[0] = pospivsign (1 = even number of swaps)
 int pivsign(la_ctx *ctx, const int64_t *d_ipiv, int64_t kmax, int32_t *d_info);
 

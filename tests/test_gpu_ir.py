@@ -108,3 +108,43 @@ def test_fp64_full_size_cfg2(tctx, torch_cuda):
     assert (r.abs().max() / (A.abs().sum(1).max() * z.abs().max())).item() < 2e-13
     eta = tctx.backward_error_dev(A, X, B)
     assert eta < 1e-14
+
+
+@pytest.mark.parametrize("prec,kappa,floor_lo,floor_hi", [
+    (la_hip.PREC_BF16, 1e2, 1e-5, 3e-2),
+    (la_hip.PREC_FP16, 1e2, 1e-6, 3e-3),
+    (la_hip.PREC_FP16, 1e3, 1e-6, 3e-3),
+])
+def test_lowprec_ir_converges(tctx, torch_cuda, prec, kappa, floor_lo, floor_hi):
+    """BF16/FP16-operand trailing updates with per-column 2^k scaling: the
+    first solve's backward error sits at the operand precision (so the
+    low-precision MFMA path is the one that ran), and FP64 refinement reaches
+    1e-15*n where kappa * u < 1."""
+    torch = torch_cuda
+    n = 1500
+    A = torch.empty((n, n), dtype=torch.float64, device="cuda")
+    b = torch.empty((n, 1), dtype=torch.float64, device="cuda")
+    xs = torch.empty_like(b)
+    tctx.gen_svd_matrix_dev(n, kappa, 16, 21, A, b=b, xstar=xs)
+    X = torch.empty_like(b)
+    rep = tctx.gesv_ir_dev(A, b, X, prec, 1e-15 * n, 200)
+    assert floor_lo < rep.initial_backward_error < floor_hi, rep.initial_backward_error
+    assert rep.status == la_hip.LA_OK, (rep.status, rep.backward_error, rep.iterations)
+    eta = tctx.backward_error_dev(A, X, b)
+    assert eta <= 1e-15 * n
+    assert ((X - xs).abs().max() / xs.abs().max()).item() <= 10 * kappa * max(eta, 1.1e-16)
+
+
+def test_bf16_ir_reports_non_convergence(tctx, torch_cuda):
+    """kappa * u_bf16 >> 1: plain refinement cannot converge; the solver must
+    say so instead of returning a silently inaccurate x."""
+    torch = torch_cuda
+    n = 1000
+    A = torch.empty((n, n), dtype=torch.float64, device="cuda")
+    b = torch.empty((n, 1), dtype=torch.float64, device="cuda")
+    xs = torch.empty_like(b)
+    tctx.gen_svd_matrix_dev(n, 1e6, 16, 22, A, b=b, xstar=xs)
+    X = torch.empty_like(b)
+    rep = tctx.gesv_ir_dev(A, b, X, la_hip.PREC_BF16, 1e-15 * n, 10)
+    assert rep.status in (la_hip.LA_NOT_CONVERGED, la_hip.LA_SINGULAR)
+    assert not (rep.backward_error <= 1e-15 * n)
