@@ -58,6 +58,60 @@ def cpu_baseline(n_sample):
             "seconds": dt}
 
 
+def batched_cholesky(ctx, steps, warmup, count=20000, seed=20000):
+    """The batched uneven Cholesky workload: `count` SPD problems, n_i uniform
+    in [8, 256], A_i = G_i G_i^T / n_i + I, one normal RHS each; potrf plus
+    both triangular solves in one graph-replayed persistent launch."""
+    import numpy as np
+    import torch
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(8, 257, size=count).astype(np.int32)
+    n64 = sizes.astype(np.int64)
+    offA = np.concatenate([[0], np.cumsum(n64 ** 2)[:-1]])
+    offB = np.concatenate([[0], np.cumsum(n64)[:-1]])
+    dn, dOA, dOB = (torch.from_numpy(x).cuda() for x in (sizes, offA, offB))
+    A = torch.empty(int((n64 ** 2).sum()), dtype=torch.float64, device="cuda")
+    G = torch.empty_like(A)
+    ctx.gen_spd_batch_dev(dn, dOA, 4, 1000, A, G)
+    del G
+    B = torch.empty(int(n64.sum()), dtype=torch.float64, device="cuda")
+    ctx.fill_normal_dev(4, 7, B)
+    L, X = torch.empty_like(A), torch.empty_like(B)
+    st = torch.empty(count, dtype=torch.int32, device="cuda")
+    ctx.autotune(0, 8, 256)
+    plan = ctx.batch_plan(0, sizes)
+    for _ in range(max(1, warmup)):
+        ctx.potrf_potrs_batched_dev(plan, dOA, dOB, A, L, B, X, st)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        ctx.potrf_potrs_batched_dev(plan, dOA, dOB, A, L, B, X, st)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    failed = int((st != 0).sum().item())
+    nbytes = float((8 * (2 * n64 ** 2 + 2 * n64)).sum())
+    flop = float((n64 ** 3 / 3 + n64 ** 2 / 2 + n64 / 6 + 4 * n64 ** 2).sum())
+    # spot-check a spread sample of backward errors on the host
+    a, b, x = A.cpu().numpy(), B.cpu().numpy(), X.cpu().numpy()
+    worst = 0.0
+    for i in range(0, count, 97):
+        n = int(sizes[i])
+        ai = a[offA[i]:offA[i] + n * n].reshape(n, n)
+        xi, bi = x[offB[i]:offB[i] + n], b[offB[i]:offB[i] + n]
+        eta = np.abs(bi - ai @ xi).max() / (np.abs(ai).sum(1).max() * np.abs(xi).max()
+                                            + np.abs(bi).max())
+        worst = max(worst, eta)
+    desc = plan.describe()
+    plan.close()
+    return {"count": count, "n_range": [8, 256], "ms": ms, "solves_per_s": count / (ms * 1e-3),
+            "hbm_gbs": nbytes / (ms * 1e-3) / 1e9,
+            "frac_of_hbm_roofline": nbytes / (ms * 1e-3) / 1e9 / PEAKS["hbm_gbs"],
+            "fp64_tflops": flop / (ms * 1e-3) / 1e12, "failed": failed,
+            "max_backward_error_sampled": worst, "plan": desc}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,6 +121,7 @@ def main():
     ap.add_argument("--kappa", type=float, default=1e6)
     ap.add_argument("--cpu-n", type=int, default=2048)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); every GPU box here has "
@@ -182,6 +237,8 @@ def main():
         "kernel_breakdown_ms": {k: round(v[1], 3) for k, v in stats.items()},
         "wall_s_timed": wall,
     }
+    if not args.no_batched:
+        out["batched_cholesky"] = batched_cholesky(ctx, args.steps, args.warmup)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_n)
         out["cpu_baseline"]["host_nproc"] = os.cpu_count()

@@ -210,9 +210,11 @@ int la_dgesv_ir_rm(la_ctx *ctx, int64_t n, int64_t nrhs, const double *A,
 typedef struct la_batch_plan la_batch_plan;
 
 /* Build a plan for a batch (host copies of the sizes are required to sort
- * and bucket the problems).  kind: 0 = Cholesky solve (FP64),
- * 1 = LU-IR solve (prec = la_prec).  The plan owns its device queue and
- * graph. */
+ * and bucket the problems).  kind 0 = Cholesky solve (FP64; n <= 256,
+ * nrhs 1, h_nrhs may be NULL); kind 1 = LU solve with FP64 refinement
+ * (n <= 1024, 1 <= nrhs <= 8, prec = LA_PREC_FP64 / FP32 / FP16 working
+ * precision).  The plan owns its device queue, the working copies of a
+ * kind-1 batch and the captured graph. */
 int la_batch_plan_create(la_ctx *ctx, int kind, int64_t count,
                          const int32_t *h_n, const int32_t *h_nrhs, int prec,
                          la_batch_plan **out);
@@ -230,6 +232,21 @@ int la_dpotrf_potrs_batched_dev(la_ctx *ctx, la_batch_plan *plan,
                                 const int64_t *d_offA, const int64_t *d_offB,
                                 const double *A, double *L,
                                 const double *B, double *X, int32_t *d_status);
+
+/* Batched LU with partial pivoting in the plan's working precision and FP64
+ * iterative refinement, one persistent launch (graph-replayed):
+ * X_i (n_i x nrhs_i at X + offB[i]) solves A_i X_i = B_i to normwise
+ * backward error tol (max over the problem's columns).  d_status[i] =
+ * LA_OK / LA_NOT_CONVERGED / LA_SINGULAR, d_iters[i] = refinement steps,
+ * d_berr[i] = final eta.  With LA_PREC_FP64 the factorisation is plain FP64
+ * LU, one solve, eta reported (tol and max_iter unused).  Each problem is
+ * LUDecomposition::new + solve (src/decomp/lu.rs:104-278) with refinement
+ * added; no batched counterpart in the reference. */
+int la_dgesv_ir_batched_dev(la_ctx *ctx, la_batch_plan *plan,
+                            const int64_t *d_offA, const int64_t *d_offB,
+                            const double *A, const double *B, double *X,
+                            double tol, int max_iter, int32_t *d_status,
+                            int32_t *d_iters, double *d_berr);
 
 /* Run the tile autotuner for `kind` over the size buckets of [nmin, nmax]
  * and store the winners in `cache_path` (JSON); later plans read it.
@@ -265,6 +282,18 @@ int la_gen_svd_matrix_dev(la_ctx *ctx, int64_t n, double kappa, int nrefl,
 int la_gen_spd_batch_dev(la_ctx *ctx, int64_t count, const int32_t *d_n,
                          const int64_t *d_offA, uint64_t seed, uint64_t stream_base,
                          double *A, double *G_scratch);
+
+/* Batch of general test systems: A_i = U_i diag(sigma) V_i^T (n_i x n_i,
+ * U_i, V_i products of nrefl <= 128 Householder reflectors, sigma_k =
+ * kappa^(-k/(n_i-1))), B_i standard normal (n_i x nrhs_i) and the exact
+ * solutions Xstar_i = V_i diag(1/sigma) U_i^T B_i.  Problem p draws its
+ * reflectors from streams stream_base + 256p + k (U) and
+ * stream_base + 256p + 128 + k (V), and B_p from stream stream_base + 256p
+ * + 255.  n_i <= 1024, nrhs_i <= 8; all pointers are device. */
+int la_gen_svd_batch_dev(la_ctx *ctx, int64_t count, const int32_t *d_n,
+                         const int32_t *d_nrhs, const int64_t *d_offA,
+                         const int64_t *d_offB, double kappa, int nrefl, uint64_t seed,
+                         uint64_t stream_base, double *A, double *B, double *Xstar);
 
 /* ------------------------------------------------------------ utilities */
 /* eta = ||B - A X||_inf / (||A||_inf ||X||_inf + ||B||_inf), FP64 on device. */

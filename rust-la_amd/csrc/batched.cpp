@@ -23,6 +23,13 @@ int launch_chol_batched(la_ctx *ctx, hipStream_t s, int grid, int count, const i
                         uint32_t *queue, const int nb[4]);
 int chol_batched_grid(la_ctx *ctx);
 int fill_spd_batch(la_ctx *ctx, int count, const int32_t *nvec, const int64_t *offA, double *A);
+int lu_ir_batched_grid(la_ctx *ctx, int prec);
+int launch_lu_ir_batched(la_ctx *ctx, hipStream_t s, int prec, int grid, int count,
+                         const int32_t *order, const int32_t *nvec, const int32_t *rhsvec,
+                         const int64_t *offA, const int64_t *offB, const int64_t *offW,
+                         const int64_t *offN, const double *A, const double *B, double *X,
+                         void *Wbuf, int32_t *permbuf, double tol, int max_iter, int32_t *status,
+                         int32_t *iters, double *berr, uint32_t *queue);
 
 }  // namespace lahip
 
@@ -35,19 +42,32 @@ struct la_batch_plan {
     int prec = 0;
     int32_t *d_order = nullptr, *d_n = nullptr, *d_nrhs = nullptr;
     uint32_t *d_queue = nullptr;
+    // LU-IR plans: working copies W_i (offsets d_offW, elements of the
+    // working precision) and pivot orders (offsets d_offN)
+    int64_t *d_offW = nullptr, *d_offN = nullptr;
+    void *d_W = nullptr;
+    int32_t *d_perm = nullptr;
     int nb[4] = {16, 16, 32, 32};
     int grid = 0;
     std::string desc;
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    const void *key[8] = {nullptr};
+    const void *key[12] = {nullptr};
 };
 
 static const int kCholCandidates[] = {8, 16, 32};
 
 static void describe(la_batch_plan *p) {
     char buf[256];
+    if (p->kind == 1) {
+        static const char *pn[] = {"fp64", "fp32", "bf16", "fp16"};
+        snprintf(buf, sizeof buf,
+                 "kind=lu-ir count=%lld grid=%d working precision=%s panel width=%d",
+                 (long long)p->count, p->grid, pn[p->prec], p->prec == LA_PREC_FP64 ? 16 : 32);
+        p->desc = buf;
+        return;
+    }
     snprintf(buf, sizeof buf,
              "kind=%s count=%lld grid=%d panel widths by bucket [<=32, <=64, <=128, <=256] = "
              "%d/%d/%d/%d%s",
@@ -56,11 +76,71 @@ static void describe(la_batch_plan *p) {
     p->desc = buf;
 }
 
+static int create_lu_ir_plan(la_ctx *ctx, int64_t count, const int32_t *h_n,
+                             const int32_t *h_nrhs, int prec, la_batch_plan **out)
+{
+    *out = nullptr;
+    if (!h_nrhs) return set_error(ctx, LA_ERR_ARG, "batched LU-IR: nrhs array required");
+    if (prec != LA_PREC_FP64 && prec != LA_PREC_FP32 && prec != LA_PREC_FP16)
+        return set_error(ctx, LA_ERR_ARG, "batched LU-IR: precision must be FP64, FP32 or FP16");
+    for (int64_t i = 0; i < count; ++i) {
+        if (h_n[i] <= 0 || h_n[i] > 1024)
+            return set_error(ctx, LA_ERR_ARG, "batched LU-IR: order must be in [1, 1024]");
+        if (h_nrhs[i] <= 0 || h_nrhs[i] > 8)
+            return set_error(ctx, LA_ERR_ARG, "batched LU-IR: nrhs must be in [1, 8]");
+    }
+    la_batch_plan *p = new la_batch_plan();
+    p->ctx = ctx;
+    p->kind = 1;
+    p->count = count;
+    p->prec = prec;
+    std::vector<int32_t> order(count);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int32_t a, int32_t b) { return h_n[a] > h_n[b]; });
+    std::vector<int64_t> offW(count), offN(count);
+    int64_t w = 0, q = 0;
+    for (int64_t i = 0; i < count; ++i) {
+        offW[i] = w;
+        offN[i] = q;
+        w += (int64_t)h_n[i] * h_n[i];
+        q += h_n[i];
+    }
+    const size_t esz = prec == LA_PREC_FP64 ? 8 : 4;
+    auto fail = [&](hipError_t e, const char *what) {
+        la_batch_plan_destroy(p);
+        return hip_fail(ctx, e, what);
+    };
+    hipError_t e;
+    if ((e = hipMalloc(&p->d_order, count * 4)) != hipSuccess) return fail(e, "plan alloc");
+    if ((e = hipMalloc(&p->d_n, count * 4)) != hipSuccess) return fail(e, "plan alloc");
+    if ((e = hipMalloc(&p->d_nrhs, count * 4)) != hipSuccess) return fail(e, "plan alloc");
+    if ((e = hipMalloc(&p->d_offW, count * 8)) != hipSuccess) return fail(e, "plan alloc");
+    if ((e = hipMalloc(&p->d_offN, count * 8)) != hipSuccess) return fail(e, "plan alloc");
+    if ((e = hipMalloc(&p->d_W, (size_t)w * esz)) != hipSuccess) return fail(e, "plan alloc");
+    if ((e = hipMalloc(&p->d_perm, (size_t)q * 4)) != hipSuccess) return fail(e, "plan alloc");
+    if ((e = hipMalloc(&p->d_queue, 256)) != hipSuccess) return fail(e, "plan alloc");
+    if ((e = hipMemcpy(p->d_order, order.data(), count * 4, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(p->d_n, h_n, count * 4, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(p->d_nrhs, h_nrhs, count * 4, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(p->d_offW, offW.data(), count * 8, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(p->d_offN, offN.data(), count * 8, hipMemcpyHostToDevice)) != hipSuccess)
+        return fail(e, "plan upload");
+    if ((e = hipMemset(p->d_queue, 0, 256)) != hipSuccess) return fail(e, "plan queue");
+    if ((e = hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking)) != hipSuccess)
+        return fail(e, "plan stream");
+    p->grid = lu_ir_batched_grid(ctx, prec);
+    describe(p);
+    *out = p;
+    return LA_OK;
+}
+
 extern "C" int la_batch_plan_create(la_ctx *ctx, int kind, int64_t count, const int32_t *h_n,
                                     const int32_t *h_nrhs, int prec, la_batch_plan **out)
 {
     if (!ctx || !out || !h_n || count <= 0 || count > INT32_MAX) return LA_ERR_ARG;
-    if (kind != 0) return set_error(ctx, LA_ERR_ARG, "batch plan: kind not supported");
+    if (kind == 1) return create_lu_ir_plan(ctx, count, h_n, h_nrhs, prec, out);
+    if (kind != 0) return set_error(ctx, LA_ERR_ARG, "batch plan: kind must be 0 or 1");
     *out = nullptr;
     for (int64_t i = 0; i < count; ++i)
         if (h_n[i] <= 0 || h_n[i] > 256)
@@ -109,6 +189,10 @@ extern "C" void la_batch_plan_destroy(la_batch_plan *p)
     if (p->d_n) (void)hipFree(p->d_n);
     if (p->d_nrhs) (void)hipFree(p->d_nrhs);
     if (p->d_queue) (void)hipFree(p->d_queue);
+    if (p->d_offW) (void)hipFree(p->d_offW);
+    if (p->d_offN) (void)hipFree(p->d_offN);
+    if (p->d_W) (void)hipFree(p->d_W);
+    if (p->d_perm) (void)hipFree(p->d_perm);
     delete p;
 }
 
@@ -123,7 +207,7 @@ extern "C" int la_dpotrf_potrs_batched_dev(la_ctx *ctx, la_batch_plan *p, const 
 {
     if (!ctx || !p || !d_offA || !d_offB || !A || !L || !B || !X || !d_status) return LA_ERR_ARG;
     if (p->kind != 0 || p->ctx != ctx) return LA_ERR_ARG;
-    const void *key[8] = {d_offA, d_offB, A, L, B, X, d_status, nullptr};
+    const void *key[12] = {d_offA, d_offB, A, L, B, X, d_status, nullptr};
     const bool same = p->exec && memcmp(key, p->key, sizeof key) == 0;
     ProfScope ps(ctx, LA_K_BATCH, 0.0, 0.0);
     if (!same) {
@@ -133,6 +217,41 @@ extern "C" int la_dpotrf_potrs_batched_dev(la_ctx *ctx, la_batch_plan *p, const 
         int st = launch_chol_batched(ctx, p->cap_stream, p->grid, (int)p->count, p->d_order,
                                      p->d_n, d_offA, d_offB, A, L, B, X, d_status, p->d_queue,
                                      p->nb);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(p->cap_stream, &g);
+        if (st) { if (g) (void)hipGraphDestroy(g); return st; }
+        if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamEndCapture");
+        p->graph = g;
+        LA_HIP_CHECK(ctx, hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0));
+        memcpy(p->key, key, sizeof key);
+    }
+    LA_HIP_CHECK(ctx, hipGraphLaunch(p->exec, ctx->stream));
+    return LA_OK;
+}
+
+extern "C" int la_dgesv_ir_batched_dev(la_ctx *ctx, la_batch_plan *p, const int64_t *d_offA,
+                                       const int64_t *d_offB, const double *A, const double *B,
+                                       double *X, double tol, int max_iter, int32_t *d_status,
+                                       int32_t *d_iters, double *d_berr)
+{
+    if (!ctx || !p || !d_offA || !d_offB || !A || !B || !X || !d_status || !d_iters || !d_berr)
+        return LA_ERR_ARG;
+    if (p->kind != 1 || p->ctx != ctx || max_iter < 0 || !(tol >= 0.0)) return LA_ERR_ARG;
+    union { double d; const void *v; } tkey;
+    tkey.v = nullptr;
+    tkey.d = tol;
+    const void *key[12] = {d_offA, d_offB, A, B, X, d_status, d_iters, d_berr, tkey.v,
+                           (const void *)(intptr_t)max_iter, nullptr, nullptr};
+    const bool same = p->exec && memcmp(key, p->key, sizeof key) == 0;
+    ProfScope ps(ctx, LA_K_BATCH, 0.0, 0.0);
+    if (!same) {
+        if (p->exec) { (void)hipGraphExecDestroy(p->exec); p->exec = nullptr; }
+        if (p->graph) { (void)hipGraphDestroy(p->graph); p->graph = nullptr; }
+        LA_HIP_CHECK(ctx, hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeThreadLocal));
+        int st = launch_lu_ir_batched(ctx, p->cap_stream, p->prec, p->grid, (int)p->count,
+                                      p->d_order, p->d_n, p->d_nrhs, d_offA, d_offB, p->d_offW,
+                                      p->d_offN, A, B, X, p->d_W, p->d_perm, tol, max_iter,
+                                      d_status, d_iters, d_berr, p->d_queue);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(p->cap_stream, &g);
         if (st) { if (g) (void)hipGraphDestroy(g); return st; }

@@ -99,14 +99,13 @@ def load():
     L.la_ctx_kernel_stats.argtypes = [_vp, ctypes.c_int, _i64p, _dp, _dp, _dp]
     L.la_fill_uniform_dev.argtypes = [_vp, ctypes.c_uint64, ctypes.c_uint64, _i64, _vp]
     L.la_fill_normal_dev.argtypes = [_vp, ctypes.c_uint64, ctypes.c_uint64, _i64, _vp]
-    for name, args in _OPTIONAL.items():
-        if hasattr(L, name):
-            getattr(L, name).argtypes = args
+    for name, args in _SIGNATURES.items():
+        getattr(L, name).argtypes = args      # a missing export fails here, loudly
     _lib = L
     return L
 
 
-_OPTIONAL = {
+_SIGNATURES = {
     "la_dgesv_ir_dev": [_vp, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, ctypes.c_int,
                         ctypes.c_double, ctypes.c_int, ctypes.POINTER(IrReport)],
     "la_dgesv_ir_rm": [_vp, _i64, _i64, _dp, _dp, _dp, ctypes.c_int, ctypes.c_double,
@@ -124,6 +123,8 @@ _OPTIONAL = {
     "la_dgesv_ir_batched_dev": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_double,
                                 ctypes.c_int, _vp, _vp, _vp],
     "la_gen_spd_batch_dev": [_vp, _i64, _vp, _vp, ctypes.c_uint64, ctypes.c_uint64, _vp, _vp],
+    "la_gen_svd_batch_dev": [_vp, _i64, _vp, _vp, _vp, _vp, ctypes.c_double, ctypes.c_int,
+                             ctypes.c_uint64, ctypes.c_uint64, _vp, _vp, _vp],
     "la_autotune": [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p],
     "la_autotune_load": [_vp, ctypes.c_char_p],
 }
@@ -345,6 +346,22 @@ class Context:
         self._check(self.lib.la_gen_spd_batch_dev(self.h, d_n.numel(), _ptr(d_n), _ptr(d_offA),
                                                   seed, stream_base, _ptr(A), _ptr(G)),
                     "gen_spd_batch")
+
+    def gesv_ir_batched_dev(self, plan, offA, offB, A, B, X, tol, max_iter, status, iters,
+                            berr):
+        """Batched LU + FP64 refinement (plan kind 1); per-problem status,
+        refinement steps and backward error land in the device arrays."""
+        self._check(self.lib.la_dgesv_ir_batched_dev(self.h, plan.h, _ptr(offA), _ptr(offB),
+                                                     _ptr(A), _ptr(B), _ptr(X), float(tol),
+                                                     int(max_iter), _ptr(status), _ptr(iters),
+                                                     _ptr(berr)), "dgesv_ir_batched")
+
+    def gen_svd_batch_dev(self, d_n, d_nrhs, offA, offB, kappa, nrefl, seed, stream_base, A, B,
+                          Xstar):
+        self._check(self.lib.la_gen_svd_batch_dev(self.h, d_n.numel(), _ptr(d_n), _ptr(d_nrhs),
+                                                  _ptr(offA), _ptr(offB), float(kappa),
+                                                  int(nrefl), seed, stream_base, _ptr(A), _ptr(B),
+                                                  _ptr(Xstar)), "gen_svd_batch")
 
     def autotune(self, kind, nmin, nmax, cache_path=None):
         self._check(self.lib.la_autotune(self.h, kind, nmin, nmax,

@@ -26,3 +26,14 @@ def torch_cuda():
     if not torch.cuda.is_available():
         pytest.fail("GPU test run without a visible GPU")
     return torch
+
+
+@pytest.fixture(scope="module")
+def tctx(torch_cuda):
+    """A context whose stream is torch's current stream, so library work and
+    torch tensor operations are ordered without extra synchronisation."""
+    import la_hip
+    c = la_hip.Context(0)
+    c.use_torch_stream()
+    yield c
+    c.close()

@@ -13,12 +13,6 @@ import la_hip
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(scope="module")
-def tctx(torch_cuda):
-    c = la_hip.Context(0)
-    c.use_torch_stream()
-    yield c
-    c.close()
 
 
 def _pack(mats, rhs):
