@@ -20,7 +20,7 @@ namespace lahip {
 int launch_chol_batched(la_ctx *ctx, hipStream_t s, int grid, int count, const int32_t *order,
                         const int32_t *nvec, const int64_t *offA, const int64_t *offB,
                         const double *A, double *L, const double *B, double *X, int32_t *status,
-                        uint32_t *queue, const int nb[4]);
+                        uint32_t *queue, const int cfg[4]);
 int chol_batched_grid(la_ctx *ctx);
 int fill_spd_batch(la_ctx *ctx, int count, const int32_t *nvec, const int64_t *offA, double *A);
 int lu_ir_batched_grid(la_ctx *ctx, int prec);
@@ -35,6 +35,8 @@ int launch_lu_ir_batched(la_ctx *ctx, hipStream_t s, int prec, int grid, int cou
 
 using namespace lahip;
 
+static constexpr int kCholDefault = 16 | (2 << 8);
+
 struct la_batch_plan {
     la_ctx *ctx = nullptr;
     int kind = 0;
@@ -47,7 +49,8 @@ struct la_batch_plan {
     int64_t *d_offW = nullptr, *d_offN = nullptr;
     void *d_W = nullptr;
     int32_t *d_perm = nullptr;
-    int nb[4] = {16, 16, 32, 32};
+    // Cholesky tile configuration per size bucket: panel width | row tiles << 8
+    int nb[4] = {kCholDefault, kCholDefault, kCholDefault, kCholDefault};
     int grid = 0;
     std::string desc;
     hipStream_t cap_stream = nullptr;
@@ -56,7 +59,7 @@ struct la_batch_plan {
     const void *key[12] = {nullptr};
 };
 
-static const int kCholCandidates[] = {8, 16, 32};
+static const int kCholCandidates[] = {8 | (1 << 8), 8 | (2 << 8), 16 | (1 << 8), 16 | (2 << 8)};
 
 static void describe(la_batch_plan *p) {
     char buf[256];
@@ -69,10 +72,11 @@ static void describe(la_batch_plan *p) {
         return;
     }
     snprintf(buf, sizeof buf,
-             "kind=%s count=%lld grid=%d panel widths by bucket [<=32, <=64, <=128, <=256] = "
-             "%d/%d/%d/%d%s",
-             p->kind == 0 ? "cholesky" : "lu-ir", (long long)p->count, p->grid, p->nb[0],
-             p->nb[1], p->nb[2], p->nb[3], p->ctx->chol_tuned ? " (autotuned)" : " (defaults)");
+             "kind=cholesky count=%lld grid=%d (panel width, row tiles) by bucket [<=32, <=64, "
+             "<=128, <=256] = (%d,%d) (%d,%d) (%d,%d) (%d,%d)%s",
+             (long long)p->count, p->grid, p->nb[0] & 255, p->nb[0] >> 8, p->nb[1] & 255,
+             p->nb[1] >> 8, p->nb[2] & 255, p->nb[2] >> 8, p->nb[3] & 255, p->nb[3] >> 8,
+             p->ctx->chol_tuned ? " (autotuned)" : " (defaults)");
     p->desc = buf;
 }
 
@@ -272,7 +276,7 @@ extern "C" int la_autotune(la_ctx *ctx, int kind, int nmin, int nmax, const char
     if (!ctx || kind != 0 || nmin < 1 || nmax > 256 || nmin > nmax) return LA_ERR_ARG;
     const int lo[4] = {1, 33, 65, 129}, hi[4] = {32, 64, 128, 256};
     int best[4];
-    memcpy(best, ctx->chol_tuned ? ctx->chol_nb : (const int[4]){16, 16, 32, 32}, sizeof best);
+    for (int b = 0; b < 4; ++b) best[b] = ctx->chol_tuned ? ctx->chol_nb[b] : kCholDefault;
     hipEvent_t e0, e1;
     LA_HIP_CHECK(ctx, hipEventCreate(&e0));
     LA_HIP_CHECK(ctx, hipEventCreate(&e1));
@@ -338,8 +342,11 @@ extern "C" int la_autotune(la_ctx *ctx, int kind, int nmin, int nmax, const char
     if (cache_path) {
         FILE *f = fopen(cache_path, "w");
         if (!f) return set_error(ctx, LA_ERR_ARG, "autotune: cannot write cache");
-        fprintf(f, "{\"kind\": \"cholesky\", \"buckets\": [32, 64, 128, 256], \"panel_width\": [%d, %d, %d, %d]}\n",
-                best[0], best[1], best[2], best[3]);
+        fprintf(f,
+                "{\"kind\": \"cholesky\", \"buckets\": [32, 64, 128, 256], "
+                "\"panel_width\": [%d, %d, %d, %d], \"row_tiles\": [%d, %d, %d, %d]}\n",
+                best[0] & 255, best[1] & 255, best[2] & 255, best[3] & 255, best[0] >> 8,
+                best[1] >> 8, best[2] >> 8, best[3] >> 8);
         fclose(f);
     }
     return LA_OK;
@@ -355,13 +362,21 @@ extern "C" int la_autotune_load(la_ctx *ctx, const char *cache_path)
     fclose(f);
     buf[nread] = 0;
     const char *k = strstr(buf, "\"panel_width\"");
-    int v[4];
-    if (!k || sscanf(strchr(k, '['), "[%d, %d, %d, %d]", &v[0], &v[1], &v[2], &v[3]) != 4)
+    int v[4], t[4] = {2, 2, 2, 2};
+    if (!k || !strchr(k, '[') ||
+        sscanf(strchr(k, '['), "[%d, %d, %d, %d]", &v[0], &v[1], &v[2], &v[3]) != 4)
         return set_error(ctx, LA_ERR_ARG, "autotune: malformed cache");
-    for (int i = 0; i < 4; ++i)
-        if (v[i] != 8 && v[i] != 16 && v[i] != 32)
+    const char *kt = strstr(buf, "\"row_tiles\"");     // optional, default 2
+    if (kt && (!strchr(kt, '[') ||
+               sscanf(strchr(kt, '['), "[%d, %d, %d, %d]", &t[0], &t[1], &t[2], &t[3]) != 4))
+        return set_error(ctx, LA_ERR_ARG, "autotune: malformed cache");
+    for (int i = 0; i < 4; ++i) {
+        if (v[i] != 8 && v[i] != 16)
             return set_error(ctx, LA_ERR_ARG, "autotune: bad panel width in cache");
-    memcpy(ctx->chol_nb, v, sizeof v);
+        if (t[i] != 1 && t[i] != 2)
+            return set_error(ctx, LA_ERR_ARG, "autotune: bad row-tile count in cache");
+    }
+    for (int i = 0; i < 4; ++i) ctx->chol_nb[i] = v[i] | (t[i] << 8);
     ctx->chol_tuned = true;
     return LA_OK;
 }

@@ -1,233 +1,314 @@
 // Batched uneven Cholesky factor + solve, FP64, one persistent launch.
 //
 // Each problem i is CholeskyDecomposition::new(A_i) followed by .solve(b_i)
-// (src/decomp/cholesky.rs:56-144): exact-symmetry check, d <= 0 rejection,
-// full L with the strict upper triangle zeroed, forward then backward
-// substitution.  Problems (order <= 256) are pulled from an atomic queue in
-// descending size order by a grid of persistent workgroups; the last
-// workgroup to finish resets the queue, so a captured graph can be replayed
-// without any host-side reset.
+// (src/decomp/cholesky.rs:56-144): exact-symmetry check, d <= 0 rejection
+// (NaN is not rejected), full L with the strict upper triangle zeroed,
+// forward then backward substitution.  Problems (order <= 256) are pulled
+// from an atomic queue in descending size order by a grid of persistent
+// workgroups; the last workgroup to finish resets the queue, so a captured
+// graph can be replayed without any host-side reset.
 //
-// L is both the output and the left-looking operand: a workgroup re-reads
-// values it wrote earlier, so L goes out with device-coherent stores and is
-// read back with device-coherent loads (a plain load could hit an L1 line
-// cached before the write: lines straddle rows of odd-sized matrices).
-//
-// Per problem (one workgroup, left-looking blocked, panel width NB):
-//   for each block column J:
-//     load panel A[J0:n, J0:J0+jb] into LDS, check it against its mirror
-//     subtract L[J0:n, 0:J0] L[J0:J0+jb, 0:J0]^T (staged chunks of NB)
-//     factor the jb x jb diagonal block, solve the rows below against it
-//     write the panel (and the zeros right of the diagonal block) to L
-//     forward-substitute this block of y and update b below it
-//   backward substitution block by block, re-reading L from L2
+// Per problem, one 256-thread workgroup, left-looking in panels of NB
+// columns (NB = 8 or 16, chosen per size bucket by the autotuner):
+//   1. P = A[J0:n, J0:J0+NB] - L[J0:n, 0:J0] L[J0:J0+NB, 0:J0]^T on
+//      v_mfma_f64_16x16x4 straight from the L written so far, A read in
+//      the accumulator layout, every strictly-lower element compared with
+//      its mirror; P goes to LDS;
+//   2. one wave factors the NB x NB diagonal block: lane r holds row r, the
+//      column being eliminated is always register 0 (rows rotate left one
+//      step per column) and column values are broadcast with readlane, so
+//      the block needs no barrier;
+//   3. every row below solves against the diagonal block in registers
+//      (row per thread), the panel and the zeros right of the diagonal block
+//      go to L, wave 0 solves this block of y, the rows below update their
+//      part of b from registers;
+//   backward substitution block by block: the rows below contribute through
+//   a coalesced column sweep of L, then one wave solves the transposed
+//   diagonal block with readlane broadcasts.
+// L is both output and left-looking operand; it is written and re-read by
+// the same workgroup with a workgroup barrier in between.
 #include "la_common.hpp"
 #include <cmath>
 
 namespace lahip {
 
-constexpr int BC_MAXN = 256;
-
-constexpr int BC_NBMAX = 32;     // LDS is sized for the widest panel
-constexpr int BC_KC = 16;        // K chunk of the left-looking update
-constexpr int BC_NBUCKET = 4;    // size buckets: [1,32], (32,64], (64,128], (128,256]
+constexpr int CB_MAXN = 256;
+constexpr int CB_NT = 256;
+constexpr int CB_NBMAX = 16;
+constexpr int CB_NBUCKET = 4;   // size buckets: [1,32], (32,64], (64,128], (128,256]
 
 __host__ __device__ inline int chol_bucket(int n) {
     return n <= 32 ? 0 : (n <= 64 ? 1 : (n <= 128 ? 2 : 3));
 }
 
-// nb: panel width of this problem's size bucket (autotuned, 8..32)
-__device__ void chol_solve_one(int n, int NB, const double *__restrict__ Ap, double *Lp,
+struct CbLds {
+    double P[CB_MAXN][CB_NBMAX + 1];   // current panel, rows J0..n-1
+    double y[CB_MAXN];                 // b -> y -> x
+    double red[CB_NT];                 // backward partial sums, [row group][column]
+    double dinv[CB_NBMAX];             // reciprocals of the diagonal block's pivots
+    int bad, prob;
+};
+
+__device__ inline double cb_bcast(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// NB: panel width; RTW: row tiles whose MFMA K loops run together (more
+// loads in flight per round trip against more registers)
+template <int NB, int CB_RTW>
+__device__ void chol_solve_one(int n, const double *__restrict__ Ap, double *Lp,
                                const double *__restrict__ bp, double *xp, int32_t *status,
-                               double (*Pn)[BC_NBMAX + 1], double (*Lk)[BC_KC + 1], double *bv,
-                               int *s_bad)
+                               CbLds &S)
 {
-    const int tid = threadIdx.x;
-    const int nthr = blockDim.x;
-    if (tid == 0) *s_bad = 0;
-    for (int i = tid; i < n; i += nthr) bv[i] = bp[i];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int l16 = lane & 15, lhi = lane >> 4;
+    if (tid == 0) S.bad = 0;
+    for (int i = tid; i < n; i += CB_NT) S.y[i] = bp[i];
     __syncthreads();
 
     for (int J0 = 0; J0 < n; J0 += NB) {
-        const int jb = min(NB, n - J0);
-        const int rows = n - J0;
-        // (a) panel + symmetry check against the mirror (row J0+c, cols > J0+c)
+        const int jb = min(NB, n - J0), m = n - J0;
+        // ---- 1. panel update on MFMA, symmetry check, P -> LDS
+        // wave w owns row tiles w, w+4, w+8, w+12 (m <= 256) and runs the
+        // K loops of CB_RTW of them together (4 + 4*CB_RTW independent loads
+        // per lane per 16-wide chunk)
+        const int rt_n = (m + 15) >> 4;
         int asym = 0;
-        for (int e = tid; e < rows * NB; e += nthr) {
-            const int r = e / NB, c = e - r * NB;
-            double v = 0.0;
-            if (c < jb) {
-                v = Ap[(int64_t)(J0 + r) * n + J0 + c];
-                if (r > c) {
-                    const double m = Ap[(int64_t)(J0 + c) * n + J0 + r];
-                    if (v != m) asym = 1;     // exact test, NaN included
+        const bool okB = l16 < jb;
+        const double *pb = Lp + (int64_t)(okB ? J0 + l16 : 0) * n + 4 * lhi;
+        for (int pass = 0; pass < 4 / CB_RTW; ++pass) {
+            if (wave + 4 * CB_RTW * pass >= rt_n) break;
+            d4_t acc[CB_RTW];
+#pragma unroll
+            for (int i = 0; i < CB_RTW; ++i) acc[i] = (d4_t){0.0, 0.0, 0.0, 0.0};
+            if (J0 > 0) {
+                const double *pa[CB_RTW];
+                bool okA[CB_RTW];
+#pragma unroll
+                for (int i = 0; i < CB_RTW; ++i) {
+                    const int rowA = J0 + 16 * (wave + 4 * (CB_RTW * pass + i)) + l16;
+                    okA[i] = rowA < n;
+                    pa[i] = Lp + (int64_t)(okA[i] ? rowA : 0) * n + 4 * lhi;
+                }
+                // k = k0 + 4*(lane/16) + s for MFMA step s, the same for A and B;
+                // J0 is a multiple of 8, so a K chunk of 16 may end inside the
+                // current panel (or past J0): those columns are masked out.
+                // KU chunks are loaded together before their MFMAs.
+                constexpr int KU = 4 / CB_RTW;
+                for (int k0 = 0; k0 < J0; k0 += 16 * KU) {
+                    double a[KU][CB_RTW][4], b[KU][4];
+#pragma unroll
+                    for (int u = 0; u < KU; ++u) {
+                        const int kk = k0 + 16 * u;
+                        const bool kin = kk + 4 * lhi < J0;
+#pragma unroll
+                        for (int s = 0; s < 4; ++s) b[u][s] = (okB && kin) ? pb[kk + s] : 0.0;
+#pragma unroll
+                        for (int i = 0; i < CB_RTW; ++i)
+#pragma unroll
+                            for (int s = 0; s < 4; ++s)
+                                a[u][i][s] = (okA[i] && kin) ? pa[i][kk + s] : 0.0;
+                    }
+#pragma unroll
+                    for (int u = 0; u < KU; ++u)
+#pragma unroll
+                        for (int i = 0; i < CB_RTW; ++i)
+#pragma unroll
+                            for (int s = 0; s < 4; ++s)
+                                acc[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u][i][s], b[u][s],
+                                                                              acc[i], 0, 0, 0);
                 }
             }
-            Pn[r][c] = v;
+#pragma unroll
+            for (int i = 0; i < CB_RTW; ++i) {
+                const int rt = wave + 4 * (CB_RTW * pass + i);
+                if (rt < rt_n) {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int pr = 16 * rt + lhi + 4 * r;      // f64 MFMA row layout
+                        if (pr < m && l16 < jb) {
+                            const int gr = J0 + pr, gc = J0 + l16;
+                            const double v = Ap[(int64_t)gr * n + gc];
+                            if (gr > gc && !(v == Ap[(int64_t)gc * n + gr])) asym = 1;   // NaN too
+                            S.P[pr][l16] = v - acc[i][r];
+                        }
+                    }
+                }
+            }
         }
-        if (asym) atomicOr(s_bad, 1);
+        if (asym) atomicOr(&S.bad, 1);
         __syncthreads();
-        // (b) left-looking update with the finished block columns:
-        //     Pn -= L[J0:n, 0:J0] * L[J0:J0+jb, 0:J0]^T, K staged in chunks of
-        //     BC_KC; thread (rg, cg) accumulates rows rg+32i (i < 8) x
-        //     columns 4cg..4cg+3 in registers.
-        if (J0 > 0) {
-            const int cg = tid & 7, rg = tid >> 3;
-            double acc[8][4];
+        // ---- 2. diagonal block, one wave, rows in registers (rotated)
+        if (wave == 0) {
+            double a[NB];
 #pragma unroll
-            for (int i = 0; i < 8; ++i)
+            for (int c = 0; c < NB; ++c) a[c] = (lane < jb && c < jb) ? S.P[lane][c] : 0.0;
+#pragma unroll 1
+            for (int t = 0; t < NB; ++t) {
+                if (t < jb) {
+                    const double dd = cb_bcast(a[0], t);      // A[t][t] after the updates
+                    if (lane == 0 && dd <= 0.0) atomicOr(&S.bad, 2);
+                    const double d = sqrt(dd);
+                    const double rd = 1.0 / d;
+                    if (lane == 0) S.dinv[t] = rd;
+                    const double l = a[0] * rd;
+                    if (lane == t) a[0] = d;
+                    else if (lane > t) a[0] = l;
+                    // columns t+1..: lane r >= t+j subtracts L[r][t] L[t+j][t];
+                    // readlane indices stay below 32, masked lanes add zero
 #pragma unroll
-                for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
-            for (int k0 = 0; k0 < J0; k0 += BC_KC) {
-                const int kc = min(BC_KC, J0 - k0);
-                for (int e = tid; e < rows * BC_KC; e += nthr) {
-                    const int r = e / BC_KC, kk = e - r * BC_KC;
-                    Lk[r][kk] = (kk < kc) ? ld_dev(Lp + (int64_t)(J0 + r) * n + k0 + kk) : 0.0;
-                }
-                __syncthreads();
-                if (tid < 256) {
-                    for (int kk = 0; kk < kc; ++kk) {
-                        double bcol[4], arow[8];
-#pragma unroll
-                        for (int j = 0; j < 4; ++j) bcol[j] = Lk[min(cg * 4 + j, rows - 1)][kk];
-#pragma unroll
-                        for (int i = 0; i < 8; ++i) arow[i] = Lk[min(rg + 32 * i, rows - 1)][kk];
-#pragma unroll
-                        for (int i = 0; i < 8; ++i)
-#pragma unroll
-                            for (int j = 0; j < 4; ++j) acc[i][j] = fma(arow[i], bcol[j], acc[i][j]);
+                    for (int j = 1; j < NB; ++j) {
+                        const double ltj = cb_bcast(l, t + j);
+                        const double lm = (lane >= t + j && t + j < jb) ? l : 0.0;
+                        a[j] = fma(-lm, ltj, a[j]);
                     }
                 }
-                __syncthreads();
-            }
-            if (tid < 256) {
+                const double a0 = a[0];
 #pragma unroll
-                for (int i = 0; i < 8; ++i) {
-                    const int r = rg + 32 * i;
+                for (int c = 0; c < NB - 1; ++c) a[c] = a[c + 1];
+                a[NB - 1] = a0;
+            }
+            if (lane < jb) {
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-                        const int c = cg * 4 + j;
-                        if (r < rows && c < jb && !(r < jb && c > r)) Pn[r][c] -= acc[i][j];
-                    }
-                }
-            }
-            __syncthreads();
-        }
-        // (c) diagonal block, right-looking in LDS
-        for (int t = 0; t < jb; ++t) {
-            if (tid == 0) {
-                const double d = Pn[t][t];
-                if (d <= 0.0) *s_bad |= 2;           // NaN is not rejected (d <= 0 false)
-                Pn[t][t] = sqrt(d);
-            }
-            __syncthreads();
-            const double dt = Pn[t][t];
-            // column t below the diagonal, for all panel rows
-            for (int r = t + 1 + tid; r < rows; r += nthr) Pn[r][t] = Pn[r][t] / dt;
-            __syncthreads();
-            // trailing update of the panel's remaining columns (rows > t)
-            const int nc = jb - t - 1;
-            for (int e = tid; e < (rows - t - 1) * nc; e += nthr) {
-                const int r = t + 1 + e / nc, c = t + 1 + e % nc;
-                if (r < jb && c > r) continue;
-                Pn[r][c] -= Pn[r][t] * Pn[c][t];
-            }
-            __syncthreads();
-        }
-        // (d) write the panel and the zeros right of the diagonal block
-        for (int e = tid; e < rows * NB; e += nthr) {
-            const int r = e / NB, c = e - r * NB;
-            if (c < jb)
-                st_dev(Lp + (int64_t)(J0 + r) * n + J0 + c, (r < jb && c > r) ? 0.0 : Pn[r][c]);
-        }
-        for (int e = tid; e < jb * (n - J0 - jb); e += nthr) {
-            const int r = e / (n - J0 - jb), c = J0 + jb + e % (n - J0 - jb);
-            st_dev(Lp + (int64_t)(J0 + r) * n + c, 0.0);
-        }
-        // (e) forward substitution for this block, then update b below
-        if (tid < 64) {
-            for (int t = 0; t < jb; ++t) {
-                double yt = 0.0;
-                if (tid == 0) bv[J0 + t] = bv[J0 + t] / Pn[t][t];
-                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                yt = bv[J0 + t];
-                for (int r = t + 1 + tid; r < jb; r += 64) bv[J0 + r] -= Pn[r][t] * yt;
-                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                for (int c = 0; c < NB; ++c)
+                    if (c <= lane) S.P[lane][c] = a[c];
             }
         }
         __syncthreads();
-        for (int r = jb + tid; r < rows; r += nthr) {
+        // ---- 3. rows below: solve against the diagonal block; write L
+        const int r = jb + tid;
+        double x[NB];
+#pragma unroll
+        for (int c = 0; c < NB; ++c) x[c] = 0.0;
+        if (r < m) {
+#pragma unroll
+            for (int c = 0; c < NB; ++c) {
+                if (c < jb) {
+                    double s = S.P[r][c];
+#pragma unroll
+                    for (int k = 0; k < c; ++k) s -= x[k] * S.P[c][k];
+                    x[c] = s * S.dinv[c];
+                }
+            }
+            double *lr = Lp + (int64_t)(J0 + r) * n + J0;
+#pragma unroll
+            for (int c = 0; c < NB; ++c)
+                if (c < jb) lr[c] = x[c];
+        }
+        // diagonal block rows, with the zeros of the upper triangle
+        for (int e = tid; e < jb * m; e += CB_NT) {
+            const int rr = e / m, c = e - rr * m;
+            Lp[(int64_t)(J0 + rr) * n + J0 + c] = (c < jb && c <= rr) ? S.P[rr][c] : 0.0;
+        }
+        // this block of y (wave 0): forward substitution with the diagonal block
+        if (wave == 0) {
+            double yv = lane < jb ? S.y[J0 + lane] : 0.0;
+#pragma unroll
+            for (int c = 0; c < NB; ++c) {
+                if (c < jb) {
+                    if (lane == c) yv = yv * S.dinv[c];
+                    const double yc = cb_bcast(yv, c);
+                    if (lane > c && lane < jb) yv -= S.P[lane][c] * yc;
+                }
+            }
+            if (lane < jb) S.y[J0 + lane] = yv;
+        }
+        __syncthreads();
+        if (r < m) {
             double s = 0.0;
-            for (int t = 0; t < jb; ++t) s = fma(Pn[r][t], bv[J0 + t], s);
-            bv[J0 + r] -= s;
+#pragma unroll
+            for (int c = 0; c < NB; ++c)
+                if (c < jb) s = fma(x[c], S.y[J0 + c], s);
+            S.y[J0 + r] -= s;
         }
         __syncthreads();
     }
 
-    // backward substitution L^T x = y, last block column first
+    // ---- backward substitution L^T x = y, last block first
+    constexpr int NG = CB_NT / NB;                 // row groups of the column sweep
     const int nblk = (n + NB - 1) / NB;
     for (int B = nblk - 1; B >= 0; --B) {
-        const int J0 = B * NB;
-        const int jb = min(NB, n - J0);
-        const int rows = n - J0;
-        for (int e = tid; e < rows * NB; e += nthr) {
-            const int r = e / NB, c = e - r * NB;
-            Pn[r][c] = (c < jb) ? ld_dev(Lp + (int64_t)(J0 + r) * n + J0 + c) : 0.0;
-        }
-        __syncthreads();
-        // contributions of the already-final x below this block
-        for (int c = tid; c < jb; c += nthr) {
+        const int J0 = B * NB, jb = min(NB, n - J0);
+        {
+            const int c = tid % NB, grp = tid / NB;
             double s = 0.0;
-            for (int r = jb; r < rows; ++r) s = fma(Pn[r][c], bv[J0 + r], s);
-            bv[J0 + c] -= s;
+            if (c < jb) {
+#pragma unroll 2
+                for (int rr = J0 + jb + grp; rr < n; rr += NG)
+                    s = fma(Lp[(int64_t)rr * n + J0 + c], S.y[rr], s);
+            }
+            S.red[grp * NB + c] = s;
         }
         __syncthreads();
-        if (tid < 64) {
-            for (int t = jb - 1; t >= 0; --t) {
-                if (tid == 0) bv[J0 + t] = bv[J0 + t] / Pn[t][t];
-                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                const double xt = bv[J0 + t];
-                for (int c = tid; c < t; c += 64) bv[J0 + c] -= Pn[t][c] * xt;
-                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        if (wave == 0) {
+            double colv[NB];          // lane c: column c of the diagonal block
+#pragma unroll
+            for (int rr = 0; rr < NB; ++rr)
+                colv[rr] = (lane < jb && rr < jb && rr >= lane)
+                               ? Lp[(int64_t)(J0 + rr) * n + J0 + lane] : 0.0;
+            const double dgi = lane < jb ? 1.0 / Lp[(int64_t)(J0 + lane) * n + J0 + lane] : 0.0;
+            double xv = 0.0;
+            if (lane < jb) {
+                double red = 0.0;
+                for (int g = 0; g < NG; ++g) red += S.red[g * NB + lane];
+                xv = S.y[J0 + lane] - red;
             }
+#pragma unroll
+            for (int rr = NB - 1; rr >= 0; --rr) {
+                if (rr < jb) {
+                    if (lane == rr) xv = xv * dgi;
+                    const double xr = cb_bcast(xv, rr);
+                    if (lane < rr) xv -= colv[rr] * xr;
+                }
+            }
+            if (lane < jb) S.y[J0 + lane] = xv;
         }
         __syncthreads();
     }
-    for (int i = tid; i < n; i += nthr) xp[i] = bv[i];
+    for (int i = tid; i < n; i += CB_NT) xp[i] = S.y[i];
     if (tid == 0) {
-        const int b = *s_bad;
+        const int b = S.bad;
         *status = (b & 1) ? LA_NOT_SYMMETRIC : ((b & 2) ? LA_NOT_SPD : LA_OK);
     }
     __syncthreads();
 }
 
-__global__ __launch_bounds__(256)
+__global__ __launch_bounds__(CB_NT, 4)
 void k_chol_batched(int count, const int32_t *__restrict__ order, const int32_t *__restrict__ nvec,
                     const int64_t *__restrict__ offA, const int64_t *__restrict__ offB,
                     const double *A, double *L, const double *B, double *X, int32_t *status,
-                    uint32_t *queue, int4 nb_by_bucket)
+                    uint32_t *queue, int4 cfg_by_bucket)
 {
-    __shared__ double Pn[BC_MAXN][BC_NBMAX + 1];
-    __shared__ double Lk[BC_MAXN][BC_KC + 1];
-    __shared__ double bv[BC_MAXN];
-    __shared__ int s_prob, s_bad;
+    __shared__ CbLds S;
     for (;;) {
-        if (threadIdx.x == 0) s_prob = (int)atomicAdd(&queue[0], 1u);
+        if (threadIdx.x == 0) S.prob = (int)atomicAdd(&queue[0], 1u);
         __syncthreads();
-        const int pi = s_prob;
+        const int pi = S.prob;
         __syncthreads();
         if (pi >= count) break;
         const int prob = order[pi];
         const int n = nvec[prob];
-        if (n <= 0 || n > BC_MAXN) {
+        if (n <= 0 || n > CB_MAXN) {
             if (threadIdx.x == 0) status[prob] = LA_ERR_SHAPE;
             continue;
         }
+        // tile configuration of this size bucket: panel width | row tiles << 8
         const int bk = chol_bucket(n);
-        const int nb = bk == 0 ? nb_by_bucket.x : bk == 1 ? nb_by_bucket.y
-                     : bk == 2 ? nb_by_bucket.z : nb_by_bucket.w;
-        chol_solve_one(n, nb, A + offA[prob], L + offA[prob], B + offB[prob], X + offB[prob],
-                       status + prob, Pn, Lk, bv, &s_bad);
+        const int cfg = bk == 0 ? cfg_by_bucket.x : bk == 1 ? cfg_by_bucket.y
+                      : bk == 2 ? cfg_by_bucket.z : cfg_by_bucket.w;
+        const double *Ap = A + offA[prob];
+        double *Lp = L + offA[prob];
+        const double *bp = B + offB[prob];
+        double *xp = X + offB[prob];
+        switch (cfg) {
+        case 8 | (1 << 8): chol_solve_one<8, 1>(n, Ap, Lp, bp, xp, status + prob, S); break;
+        case 8 | (2 << 8): chol_solve_one<8, 2>(n, Ap, Lp, bp, xp, status + prob, S); break;
+        case 16 | (1 << 8): chol_solve_one<16, 1>(n, Ap, Lp, bp, xp, status + prob, S); break;
+        default: chol_solve_one<16, 2>(n, Ap, Lp, bp, xp, status + prob, S); break;
+        }
     }
     // the last workgroup out resets the queue for the next launch or replay
     if (threadIdx.x == 0) {
@@ -264,8 +345,8 @@ int fill_spd_batch(la_ctx *ctx, int count, const int32_t *nvec, const int64_t *o
 int chol_batched_grid(la_ctx *ctx)
 {
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_chol_batched, 256, 0) !=
-            hipSuccess || per_cu < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_chol_batched, CB_NT,
+                                                     0) != hipSuccess || per_cu < 1)
         per_cu = 1;
     (void)hipGetLastError();
     return std::max(1, ctx->num_cus) * per_cu;
@@ -274,11 +355,11 @@ int chol_batched_grid(la_ctx *ctx)
 int launch_chol_batched(la_ctx *ctx, hipStream_t s, int grid, int count, const int32_t *order,
                         const int32_t *nvec, const int64_t *offA, const int64_t *offB,
                         const double *A, double *L, const double *B, double *X, int32_t *status,
-                        uint32_t *queue, const int nb[4])
+                        uint32_t *queue, const int cfg[4])
 {
-    const int4 nbv = make_int4(nb[0], nb[1], nb[2], nb[3]);
-    hipLaunchKernelGGL(k_chol_batched, dim3(grid), dim3(256), 0, s, count, order, nvec, offA, offB,
-                       A, L, B, X, status, queue, nbv);
+    const int4 nbv = make_int4(cfg[0], cfg[1], cfg[2], cfg[3]);
+    hipLaunchKernelGGL(k_chol_batched, dim3(grid), dim3(CB_NT), 0, s, count, order, nvec, offA,
+                       offB, A, L, B, X, status, queue, nbv);
     LA_HIP_CHECK(ctx, hipGetLastError());
     return LA_OK;
 }

@@ -129,6 +129,7 @@ __device__ void bl_convert(int n, const double *__restrict__ Ap,
         double s = 0.0;
         const double *ar = Ap + (int64_t)r * n;
         T *wr = Wp + (int64_t)r * n;
+#pragma unroll 4
         for (int c = lane; c < n; c += 64) {
             const double v = ar[c];
             wr[c] = (T)v;
@@ -325,64 +326,75 @@ __device__ void bl_getrf(int n, typename BlCfg<MODE>::T *__restrict__ W, int *__
                     ustg[tid][i] = (MODE == LA_PREC_FP16) ? (E)((float)x[i] * uscale) : (E)x[i];
             }
             __syncthreads();
+            // each wave: one 16-row tile at a time against all column tiles of
+            // the chunk, G column tiles per step with their C values loaded
+            // together so that 4*G reads per lane are in flight
             const int rt_n = (R + 15) >> 4, ct_n = (cw + 15) >> 4;
+            constexpr int G = sizeof(T) == 8 ? 4 : 8;
             for (int rt = wave; rt < rt_n; rt += 4) {
                 const int row = c_begin + 16 * rt + l16;
                 const bool rok = row < n;
                 const T *lrow = W + (int64_t)row * n + k0;
+                h8_t ah;
+                T af[NB / 4];
                 if constexpr (MODE == LA_PREC_FP16) {
-                    h8_t af;
 #pragma unroll
                     for (int e = 0; e < 8; ++e) {
                         const int k = 8 * lhi + e;
-                        af[e] = (_Float16)((rok && k < jb) ? lrow[k] : 0.0f);
+                        ah[e] = (_Float16)((rok && k < jb) ? lrow[k] : 0.0f);
                     }
-                    for (int ct = 0; ct < ct_n; ++ct) {
-                        const h8_t bfv = *reinterpret_cast<const h8_t *>(&ustg[16 * ct + l16][8 * lhi]);
-                        f4_t acc = {0.f, 0.f, 0.f, 0.f};
-                        acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(af, bfv, acc, 0, 0, 0);
-                        const int col = cc0 + 16 * ct + l16;
+                } else {
+#pragma unroll
+                    for (int s = 0; s < NB / 4; ++s) {
+                        const int k = 4 * s + lhi;
+                        af[s] = (rok && k < jb) ? lrow[k] : (T)0;
+                    }
+                }
+                for (int ct0 = 0; ct0 < ct_n; ct0 += G) {
+                    T cv[G][4];
+#pragma unroll
+                    for (int g = 0; g < G; ++g) {
+                        const int col = cc0 + 16 * (ct0 + g) + l16;
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const int crow = c_begin + 16 * rt + BlMfma<MODE>::drow(lhi, r);
+                            cv[g][r] = (crow < n && col < cc0 + cw) ? W[(int64_t)crow * n + col]
+                                                                    : (T)0;
+                        }
+                    }
+#pragma unroll
+                    for (int g = 0; g < G; ++g) {
+                        const int cl = 16 * (ct0 + g) + l16;     // < BL_CHUNK: ct < 16
+                        T d[4];
+                        if constexpr (MODE == LA_PREC_FP16) {
+                            const h8_t bfv = *reinterpret_cast<const h8_t *>(&ustg[cl][8 * lhi]);
+                            f4_t acc = {0.f, 0.f, 0.f, 0.f};
+                            acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bfv, acc, 0, 0, 0);
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) d[r] = acc[r] * uinv;
+                        } else if constexpr (MODE == LA_PREC_FP64) {
+                            d4_t acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                            for (int s = 0; s < NB / 4; ++s)
+                                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(af[s], ustg[cl][4 * s + lhi],
+                                                                           acc, 0, 0, 0);
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) d[r] = acc[r];
+                        } else {
+                            f4_t acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                            for (int s = 0; s < NB / 4; ++s)
+                                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(af[s], ustg[cl][4 * s + lhi],
+                                                                           acc, 0, 0, 0);
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) d[r] = acc[r];
+                        }
+                        const int col = cc0 + cl;
 #pragma unroll
                         for (int r = 0; r < 4; ++r) {
                             const int crow = c_begin + 16 * rt + BlMfma<MODE>::drow(lhi, r);
                             if (crow < n && col < cc0 + cw)
-                                W[(int64_t)crow * n + col] -= acc[r] * uinv;
-                        }
-                    }
-                } else {
-                    constexpr int KS = NB / 4;
-                    T af[KS];
-#pragma unroll
-                    for (int s = 0; s < KS; ++s) {
-                        const int k = 4 * s + lhi;
-                        af[s] = (rok && k < jb) ? lrow[k] : (T)0;
-                    }
-                    for (int ct = 0; ct < ct_n; ++ct) {
-                        const int cl = 16 * ct + l16;
-                        if constexpr (MODE == LA_PREC_FP64) {
-                            d4_t acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-                            for (int s = 0; s < KS; ++s)
-                                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(af[s], ustg[cl][4 * s + lhi],
-                                                                           acc, 0, 0, 0);
-                            const int col = cc0 + cl;
-#pragma unroll
-                            for (int r = 0; r < 4; ++r) {
-                                const int crow = c_begin + 16 * rt + BlMfma<MODE>::drow(lhi, r);
-                                if (crow < n && col < cc0 + cw) W[(int64_t)crow * n + col] -= acc[r];
-                            }
-                        } else {
-                            f4_t acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-                            for (int s = 0; s < KS; ++s)
-                                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(af[s], ustg[cl][4 * s + lhi],
-                                                                           acc, 0, 0, 0);
-                            const int col = cc0 + cl;
-#pragma unroll
-                            for (int r = 0; r < 4; ++r) {
-                                const int crow = c_begin + 16 * rt + BlMfma<MODE>::drow(lhi, r);
-                                if (crow < n && col < cc0 + cw) W[(int64_t)crow * n + col] -= acc[r];
-                            }
+                                W[(int64_t)crow * n + col] = cv[g][r] - d[r];
                         }
                     }
                 }
@@ -411,6 +423,7 @@ __device__ void bl_gemv_sub(int n, const T *__restrict__ W, T *Y, int g, int b0,
         for (int c = 0; c < BL_MAXRHS; ++c) acc[c] = (T)0;
         if (il < bs) {
             const T *wr = W + (int64_t)i * n;
+#pragma unroll 8
             for (int j = j0 + l16; j < j1; j += 16) {
                 const T w = wr[j];
 #pragma unroll
@@ -502,8 +515,16 @@ __device__ void bl_residual(int n, int nrhs, int c0, int g, bool first,
 {
     typedef typename BlCfg<MODE>::T T;
     T *Y = reinterpret_cast<T *>(L.y);
+    double *xs = reinterpret_cast<double *>(L.stg);      // x block, [j][g], in LDS
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int sub = lane >> 4, l16 = lane & 15;
+    if (!first) {
+        for (int e = tid; e < n * g; e += BL_NT) {
+            const int j = e / g, c = e - j * g;
+            xs[e] = Xp[(int64_t)j * nrhs + c0 + c];
+        }
+        __syncthreads();
+    }
     double rmax[BL_MAXRHS];
 #pragma unroll
     for (int c = 0; c < BL_MAXRHS; ++c) rmax[c] = 0.0;
@@ -515,12 +536,13 @@ __device__ void bl_residual(int n, int nrhs, int c0, int g, bool first,
         const int row = i < n ? perm[i] : 0;
         if (i < n && !first) {
             const double *ar = Ap + (int64_t)row * n;
+            // 8 independent row loads in flight per lane
+#pragma unroll 8
             for (int j = l16; j < n; j += 16) {
                 const double a = ar[j];
-                const double *xr = Xp + (int64_t)j * nrhs + c0;
 #pragma unroll
                 for (int c = 0; c < BL_MAXRHS; ++c)
-                    if (c < g) acc[c] = fma(a, xr[c], acc[c]);
+                    if (c < g) acc[c] = fma(a, xs[j * g + c], acc[c]);
             }
         }
         if (!first) {
@@ -598,7 +620,8 @@ __device__ void bl_refine(int n, int nrhs, const double *__restrict__ Ap,
     typedef typename BlCfg<MODE>::T T;
     T *Y = reinterpret_cast<T *>(L.y);
     const int cap = BL_Y_BYTES / (int)sizeof(T);
-    const int gmax = max(1, min(min(nrhs, BL_MAXRHS), cap / n));
+    const int xcap = BL_STG_BYTES / (int)sizeof(double);
+    const int gmax = max(1, min(min(nrhs, BL_MAXRHS), min(cap, xcap) / n));
     status = LA_OK;
     iters = 0;
     eta_out = 0.0;

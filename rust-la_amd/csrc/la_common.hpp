@@ -77,7 +77,8 @@ struct la_ctx {
     std::string last_error;
     // kernel timing harness
     uint32_t trsv_epoch = 0;        // tag of the triangular-solve exchange
-    int chol_nb[4] = {16, 16, 32, 32};   // batched Cholesky panel width per size bucket
+    // batched Cholesky tile configuration per size bucket: panel width | row tiles << 8
+    int chol_nb[4] = {16 | (2 << 8), 16 | (2 << 8), 16 | (2 << 8), 16 | (2 << 8)};
     bool chol_tuned = false;
     bool prof = false;
     std::vector<hipEvent_t> ev_pool;

@@ -44,8 +44,17 @@ def _run(tctx, torch, sizes, mats, rhs, replays=1):
     return offA, offB, outs
 
 
-def test_batched_cholesky_vs_oracle(tctx, torch_cuda):
+@pytest.mark.parametrize("nb,rt", [(8, 1), (8, 2), (16, 1), (16, 2)])
+def test_batched_cholesky_vs_oracle(tctx, torch_cuda, tmp_path, nb, rt):
+    """Every problem against the oracle, with each tile configuration the
+    autotuner can choose (panel width, row tiles per MFMA pass) forced for
+    all size buckets."""
     torch = torch_cuda
+    cache = tmp_path / "tune.json"
+    cache.write_text('{"kind": "cholesky", "buckets": [32, 64, 128, 256], '
+                     f'"panel_width": [{nb}, {nb}, {nb}, {nb}], '
+                     f'"row_tiles": [{rt}, {rt}, {rt}, {rt}]}}\n')
+    tctx.autotune_load(str(cache))
     rng = np.random.default_rng(5)
     sizes = [int(x) for x in rng.integers(1, 257, size=120)] + [1, 2, 31, 32, 33, 64, 65, 128, 129, 256]
     mats = [oracle.spd_matrix(n, 77, 1000 + i) for i, n in enumerate(sizes)]
@@ -80,8 +89,8 @@ def test_batched_failures_are_isolated(tctx, torch_cuda):
         assert np.abs(X[offB[i]:offB[i] + n] - x_o).max() <= 1e-12 * np.abs(x_o).max()
 
 
-def test_cfg4_full_batch_properties(tctx, torch_cuda):
-    """Config 4 at full size: 20000 SPD problems, n ~ U{8..256},
+def test_full_batch_20000_properties(tctx, torch_cuda):
+    """The full workload: 20000 SPD problems, n ~ U{8..256},
     A_i = G G^T / n + I generated on the GPU; every solution's normwise
     backward error is checked on the host."""
     torch = torch_cuda
