@@ -141,33 +141,32 @@ __device__ void panel_net_moves(int NB, int kglob, const int *s_piv, int64_t *sw
 
 
 // X = L^-1 for the unit lower nb x nb triangle L (LDS, row stride ldl; only
-// the strict lower part is read).  Row sweep: X[i][c] = -sum_{t=c}^{i-1}
-// L[i][t] X[t][c] for c < i, X[i][i] = 1; the 256 threads split each dot
-// four ways.  X (LDS, 64 x 65) is written to out (NB_IN x NB_IN, zero above
-// the diagonal).  Called by a whole workgroup.
+// the strict lower part is read), by ONE wavefront with no barrier: lane c
+// forms column c, X[i][c] = -sum_{t=c}^{i-1} L[i][t] X[t][c], X[c][c] = 1.
+// The sum is kept as four interleaved partial sums over t = c+q, c+q+4, ...
+// (q = 0..3) combined as (p0 + p1) + (p2 + p3): terms with t < c are exact
+// zeros, so the accumulators indexed by t mod 4 are the same partial sums
+// rotated by c mod 4.  Every L[i][t] is an LDS broadcast.  out is
+// NB_IN x NB_IN, zero outside the nb x nb triangle.
 template <typename T>
-__device__ void invert_unit_lower(const T *L, int ldl, int nb, T (*X)[65], T *out)
+__device__ void invert_unit_lower_wave(const T *L, int ldl, int nb, T *out)
 {
-    __shared__ T part[4][64];
-    const int tid = threadIdx.x, c = tid & 63, q = tid >> 6;
-    for (int e = tid; e < 64 * 64; e += blockDim.x) {
-        const int r = e >> 6, cc = e & 63;
-        X[r][cc] = (r == cc) ? T(1) : T(0);
+    const int c = threadIdx.x & 63, cm = c & 3;
+    T x[NB_IN];
+#pragma unroll
+    for (int i = 0; i < NB_IN; ++i) {
+        T acc[4] = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+        for (int t = 0; t < i; ++t) acc[t & 3] += L[i * ldl + t] * x[t];
+        // part q sums t = c+q (mod 4): accumulator (q + c) mod 4
+        const T p0 = cm == 0 ? acc[0] : cm == 1 ? acc[1] : cm == 2 ? acc[2] : acc[3];
+        const T p1 = cm == 0 ? acc[1] : cm == 1 ? acc[2] : cm == 2 ? acc[3] : acc[0];
+        const T p2 = cm == 0 ? acc[2] : cm == 1 ? acc[3] : cm == 2 ? acc[0] : acc[1];
+        const T p3 = cm == 0 ? acc[3] : cm == 1 ? acc[0] : cm == 2 ? acc[1] : acc[2];
+        x[i] = (i == c) ? T(1) : (i > c ? -((p0 + p1) + (p2 + p3)) : T(0));
     }
-    __syncthreads();
-    for (int i = 1; i < nb; ++i) {
-        T s = T(0);
-        if (c < i)
-            for (int t = c + q; t < i; t += 4) s += L[i * ldl + t] * X[t][c];
-        part[q][c] = s;
-        __syncthreads();
-        if (q == 0 && c < i) X[i][c] = -((part[0][c] + part[1][c]) + (part[2][c] + part[3][c]));
-        __syncthreads();
-    }
-    for (int e = tid; e < NB_IN * NB_IN; e += blockDim.x) {
-        const int r = e / NB_IN, cc = e - r * NB_IN;
-        out[e] = (r < nb && cc < nb) ? X[r][cc] : T(0);
-    }
+#pragma unroll
+    for (int i = 0; i < NB_IN; ++i) out[i * NB_IN + c] = (i < nb && c < nb) ? x[i] : T(0);
 }
 
 // ------------------------------------------------------------- panel kernel
@@ -482,11 +481,8 @@ void k_getrf_panel(T *A, int64_t lda, int M, int NB, int R, int kglob, int64_t *
     }
     __syncthreads();
     if (w == 0 && tid < 64) panel_net_moves(NB, kglob, s_piv, swapmap);
-    if (w == 0) {
-        // L11 (rows 0..NB-1 of the panel, all owned here since R >= NB)
-        __shared__ T Xs[64][65];
-        invert_unit_lower<T>(blk, LDB, NB, Xs, linv);
-    }
+    // L11 (rows 0..NB-1 of the panel, all owned by workgroup 0 since R >= NB)
+    if (w == 0 && wave == 1) invert_unit_lower_wave<T>(blk, LDB, NB, linv);
 }
 
 // ------------------------------------------------------ single-WG panel
@@ -579,8 +575,7 @@ void k_getrf_panel_local(T *A, int64_t lda, int M, int NB, int kglob, int64_t *i
     }
     __syncthreads();
     if (tid < 64) panel_net_moves(NB, kglob, s_piv, swapmap);
-    __shared__ T Xs[64][65];
-    invert_unit_lower<T>(blk, LDB, NB, Xs, linv);
+    if (wave == 1) invert_unit_lower_wave<T>(blk, LDB, NB, linv);
 }
 
 // ------------------------------------------------------------ row swaps
@@ -640,9 +635,11 @@ void k_apply_rowmoves(T *A, int64_t lda, int n, int c_lo, int c_hi,
 
 // ------------------------------------------------------------ unit TRSM
 // B (nb x ncols, in place, row-major) <- L^-1 B, given Linv = L^-1 (NB_IN x
-// NB_IN, produced by the panel kernel).  A workgroup stages 64 columns of B
-// and Linv in LDS; thread (column c, row group q) forms rows q, q+4, ...
-// No per-row dependency remains, so there is a single barrier.
+// NB_IN, unit diagonal stored, zero above it, produced by the panel kernel).
+// Linv and 64 columns of B staged in LDS; thread (column cl, row group rq)
+// keeps 16 independent accumulators (rows rq + 4k), so the inner loop is a
+// stream of LDS broadcasts and FMAs with no dependent chain; each row sums
+// B[i] + Linv[i][t] B[t] for t ascending below the diagonal.
 template <typename T>
 __global__ __launch_bounds__(256)
 void k_trsm_unit_lower(const T *Linv, int nb, T *B, int64_t ldb, int ncols)
@@ -652,18 +649,30 @@ void k_trsm_unit_lower(const T *Linv, int nb, T *B, int64_t ldb, int ncols)
     const int tid = threadIdx.x;
     const int c0 = blockIdx.x * 64;
     const int cl = tid & 63, rq = tid >> 6;
+    // staged with a zero diagonal: row i then sums B[i] + Linv[i][t] B[t]
+    // over t ascending, the t >= i terms adding exact zeros
     for (int e = tid; e < 64 * 64; e += 256) {
         const int i = e >> 6, t = e & 63;
-        Li[i][t] = (i < nb && t < nb) ? Linv[i * NB_IN + t] : T(0);
+        Li[i][t] = (i < nb && t < nb && t != i) ? Linv[i * NB_IN + t] : T(0);
     }
     for (int i = rq; i < 64; i += 4)
         Bs[i][cl] = (i < nb && c0 + cl < ncols) ? B[(int64_t)i * ldb + c0 + cl] : T(0);
     __syncthreads();
+    // row i = rq + 4k: s = B[i] + sum_{t<i} Linv[i][t] B[t], t ascending
+    T acc[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) acc[k] = Bs[rq + 4 * k][cl];
+#pragma unroll 8
+    for (int t = 0; t < 64; ++t) {
+        const T b = Bs[t][cl];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) acc[k] += Li[rq + 4 * k][t] * b;
+    }
     if (c0 + cl >= ncols) return;
-    for (int i = rq; i < nb; i += 4) {
-        T s = Bs[i][cl];                 // unit diagonal
-        for (int t = 0; t < i; ++t) s += Li[i][t] * Bs[t][cl];
-        B[(int64_t)i * ldb + c0 + cl] = s;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const int i = rq + 4 * k;
+        if (i < nb) B[(int64_t)i * ldb + c0 + cl] = acc[k];
     }
 }
 

@@ -108,17 +108,43 @@ void k_trsv(int n, const TF *__restrict__ Tm, int64_t ldt, int nrhs, double *X, 
     for (int r = 0; r < NR; ++r) acc[r] = 0.0;
     const int i = TRANS ? (tid & 63) : (tid >> 2);
     const int q = TRANS ? (tid >> 6) : (tid & 3);
+    // The last two dependencies (blocks t-2, t-1) form the final chunk, on
+    // the critical path: their 64 x 64 tiles of the factor and this block's
+    // slice of the right-hand side do not depend on x, so they are loaded
+    // now, before any wait.
+    const int nlast = min(t, 2);
+    TF tpre[2][16];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+#pragma unroll
+        for (int cc = 0; cc < 16; ++cc) tpre[k][cc] = TF(0);
+        if (k < nlast && i < rb) {
+            const int step = t - nlast + k;
+            const int c = FWD ? step : nblocks - 1 - step;
+            const int c0 = c * TB, cb = min(TB, n - c0);
+#pragma unroll
+            for (int cc = 0; cc < 16; ++cc) {
+                const int col = q * 16 + cc;
+                if (col < cb)
+                    tpre[k][cc] = !TRANS ? Tm[(int64_t)(r0 + i) * ldt + c0 + col]
+                                         : Tm[(int64_t)(c0 + col) * ldt + r0 + i];
+            }
+        }
+    }
+    double bpre = 0.0;
+    const int ba = tid / max(nrhs, 1), br = tid - ba * max(nrhs, 1);
+    if (tid < TB * nrhs && ba < rb) bpre = X[(int64_t)(r0 + ba) * ldx + br];
     // Blocks per chunk: small, so that a workgroup never sits on a large
     // batch of finished blocks while its immediate predecessor completes
     // (the batch right before the last dependency is on the critical path).
     const int CH = max(1, min(8, 64 / max(nrhs, 1)));
     const int words_per_block = TB * nrhs * 2;
 
-    // fold in finished blocks: all but the last dependency in chunks of CH,
-    // then the last one alone (it is the one on the critical path)
+    // fold in finished blocks: all but the last two dependencies in chunks
+    // of CH, then the last two together (the final chunk, critical path)
     int done = 0;
     while (done < t) {
-        const int chunk = (done < t - 1) ? min(CH, t - 1 - done) : 1;
+        const int chunk = (done < t - nlast) ? min(CH, t - nlast - done) : nlast;
         // gather the tagged x words of blocks done .. done+chunk-1: sixteen
         // independent loads in flight per thread, then validate; only words
         // that are not yet current are polled again
@@ -163,22 +189,45 @@ void k_trsv(int n, const TF *__restrict__ Tm, int64_t ldt, int nrhs, double *X, 
         }
         __syncthreads();
         if (i < rb) {
-            for (int k = 0; k < chunk; ++k) {
-                const int step = done + k;
-                const int c = FWD ? step : nblocks - 1 - step;
-                const int c0 = c * TB;
-                const int cb = min(TB, n - c0);
-                const double *xk = xs + (size_t)k * TB * nrhs;
-#pragma unroll 4
-                for (int cc = 0; cc < 16; ++cc) {
-                    const int col = q * 16 + cc;
-                    if (col < cb) {
-                        double tv;
-                        if (!TRANS) tv = (double)Tm[(int64_t)(r0 + i) * ldt + c0 + col];
-                        else        tv = (double)Tm[(int64_t)(c0 + col) * ldt + r0 + i];
+            if (done == t - nlast) {
+                // the final chunk: its tiles are already in registers
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    if (k < nlast) {
+                        const double *xk = xs + (size_t)k * TB * nrhs;
+#pragma unroll
+                        for (int cc = 0; cc < 16; ++cc) {
+                            const int col = q * 16 + cc;
+#pragma unroll
+                            for (int r = 0; r < NR; ++r)
+                                if (r < nrhs)
+                                    acc[r] = __builtin_fma((double)tpre[k][cc],
+                                                           xk[col * nrhs + r], acc[r]);
+                        }
+                    }
+                }
+            } else {
+                for (int k = 0; k < chunk; ++k) {
+                    const int step = done + k;
+                    const int c = FWD ? step : nblocks - 1 - step;
+                    const int c0 = c * TB;
+                    const int cb = min(TB, n - c0);
+                    const double *xk = xs + (size_t)k * TB * nrhs;
+                    double tv[16];
+#pragma unroll
+                    for (int cc = 0; cc < 16; ++cc) {
+                        const int col = q * 16 + cc;
+                        tv[cc] = 0.0;
+                        if (col < cb)
+                            tv[cc] = !TRANS ? (double)Tm[(int64_t)(r0 + i) * ldt + c0 + col]
+                                            : (double)Tm[(int64_t)(c0 + col) * ldt + r0 + i];
+                    }
+#pragma unroll
+                    for (int cc = 0; cc < 16; ++cc) {
+                        const int col = q * 16 + cc;
 #pragma unroll
                         for (int r = 0; r < NR; ++r)
-                            if (r < nrhs) acc[r] = __builtin_fma(tv, xk[col * nrhs + r], acc[r]);
+                            if (r < nrhs) acc[r] = __builtin_fma(tv[cc], xk[col * nrhs + r], acc[r]);
                     }
                 }
             }
@@ -225,7 +274,7 @@ void k_trsv(int n, const TF *__restrict__ Tm, int64_t ldt, int nrhs, double *X, 
         const int a = e / nrhs, r = e - a * nrhs;
         double y = 0.0;
         if (a < rb)
-            y = X[(int64_t)(r0 + a) * ldx + r] -
+            y = (e == tid ? bpre : X[(int64_t)(r0 + a) * ldx + r]) -
                 ((red[0][a][r] + red[1][a][r]) + (red[2][a][r] + red[3][a][r]));
         ys[a][r] = y;
     }

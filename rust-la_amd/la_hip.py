@@ -146,6 +146,15 @@ class Context:
     """One la_ctx (device, stream, workspaces)."""
 
     def __init__(self, device=0):
+        # PyTorch-ROCm bundles its own HIP runtime; when both live in one
+        # process, torch's must be initialised first or torch then reports
+        # no device.  Initialise it here when torch is importable.
+        try:
+            import torch
+            if torch.cuda.is_available():
+                torch.cuda.init()
+        except ImportError:
+            pass
         self.lib = load()
         h = _vp()
         st = self.lib.la_ctx_create(device, ctypes.byref(h))

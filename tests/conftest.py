@@ -13,7 +13,7 @@ def pytest_configure(config):
 
 
 @pytest.fixture(scope="session")
-def ctx():
+def ctx(torch_cuda):
     import la_hip
     c = la_hip.Context(0)
     yield c
