@@ -721,15 +721,18 @@ static int launch_panel(la_ctx *ctx, T *Apanel, int64_t lda, int M, int NB, int 
         LA_HIP_CHECK(ctx, hipGetLastError());
         return LA_OK;
     }
-    // Workgroups poll every peer's slot once per column with device-coherent
-    // loads, so the poll traffic grows with P^2: hold P at 64 while the
-    // rows fit (64 rows per lane-row of the first wavefront, up to 4).
+    // More, shorter workgroups shorten the per-column critical path (the
+    // rank-1 update and the candidate search scale with the rows each group
+    // holds), so a panel prefers 64 rows per workgroup (FP32) or 128 (FP64)
+    // and grows the rows per group only when more than maxp groups would be
+    // needed; the LDS holds at most 4 (FP32) or 3 (FP64) lane-rows.
     const int maxp = std::min(PANEL_MAXP, ctx->num_cus > 0 ? ctx->num_cus : 256);
-    // LDS: R x (NB+1) rows + 35 KB static (inverse workspace) must fit 160 KB
     const int rpl_max = sizeof(T) == 8 ? 3 : 4;
-    int rpl = (int)std::min<int64_t>(rpl_max, std::max<int64_t>(1, ceil_div(ceil_div(M, 64), 64)));
+    const int rpl_pref = sizeof(T) == 8 ? 2 : 1;
+    const int rpl_fit = (int)std::max<int64_t>(1, ceil_div(M, (int64_t)64 * maxp));
+    const int rpl = std::min(rpl_max, std::max(rpl_pref, std::min(rpl_fit, rpl_max)));
     int R = 64 * rpl;
-    int P = (int)ceil_div(M, R);
+    const int P = (int)ceil_div(M, R);
     if (P > maxp) return set_error(ctx, LA_ERR_ARG, "getrf panel: too many rows");
     size_t shmem = (size_t)R * (NB + 1) * sizeof(T);
     static size_t coop_max[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
