@@ -317,18 +317,36 @@ void k_getrf_panel(T *A, int64_t lda, int M, int NB, int R, int kglob, int64_t *
             uint64_t ck = 0;
             int ci = IDX_NONE;
             int abort = 0;
-            for (int q = lane; q < P; q += 64) {
-                const uint64_t *slot = cand + ((size_t)buf * PANEL_MAXP + q) * CAND_WORDS;
+            // lane q polls workgroups q, q+64, ... (P <= 256): every slot's
+            // words are loaded before any is checked, and only slots not yet
+            // current are polled again, so a pass costs one round trip
+            {
+                constexpr int SPL = PANEL_MAXP / 64;     // slots per lane
+                uint64_t w0[SPL], w1[SPL];
+                uint32_t pend = 0;
+#pragma unroll
+                for (int i = 0; i < SPL; ++i)
+                    if (lane + 64 * i < P) pend |= 1u << i;
                 const uint64_t t0 = wall_clock64();
-                for (uint32_t spin = 0;; ++spin) {
-                    const uint64_t w1 = ld_dev(slot + 1);
-                    const uint64_t w0 = ld_dev(slot + 0);
-                    if ((uint32_t)(w1 >> 32) == gs && (w0 & TAG_BIT) == tagb) {
-                        const uint64_t k = w0 & ~TAG_BIT;
-                        const int i = (int)((uint32_t)w1 & 0x7fffffffu);
-                        if (better(k, i, ck, ci)) { ck = k; ci = i; }
-                        break;
-                    }
+                for (uint32_t spin = 0; pend; ++spin) {
+#pragma unroll
+                    for (int i = 0; i < SPL; ++i)
+                        if ((pend >> i) & 1) {
+                            const uint64_t *slot =
+                                cand + ((size_t)buf * PANEL_MAXP + lane + 64 * i) * CAND_WORDS;
+                            w1[i] = ld_dev(slot + 1);
+                            w0[i] = ld_dev(slot + 0);
+                        }
+#pragma unroll
+                    for (int i = 0; i < SPL; ++i)
+                        if (((pend >> i) & 1) && (uint32_t)(w1[i] >> 32) == gs &&
+                            (w0[i] & TAG_BIT) == tagb) {
+                            pend &= ~(1u << i);
+                            const uint64_t k = w0[i] & ~TAG_BIT;
+                            const int ix = (int)((uint32_t)w1[i] & 0x7fffffffu);
+                            if (better(k, ix, ck, ci)) { ck = k; ci = ix; }
+                        }
+                    if (!pend) break;
                     if ((spin & 31) == 31 && ld_status(status) != 0) { abort = 1; break; }
                     if (wall_clock64() - t0 > spin_ticks) {
                         atomicCAS((int *)status, 0, (int)LA_ERR_TIMEOUT);
@@ -351,22 +369,38 @@ void k_getrf_panel(T *A, int64_t lda, int M, int NB, int R, int kglob, int64_t *
                 const bool need_d = (p != j) && (p >= rowbase && p < rowbase + nrows);
                 const uint64_t *wsrc = cand + ((size_t)buf * PANEL_MAXP + wp) * CAND_WORDS + 2;
                 const uint64_t *dsrc = diagw + (size_t)buf * W * NB_IN;
-                for (int wi = lane; wi < W * NB * (need_d ? 2 : 1); wi += 64) {
+                // all of a lane's words (up to 4) are loaded before any is
+                // checked, so they cost one round trip, not one each
+                const int nw = W * NB * (need_d ? 2 : 1);
+                uint64_t v[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int wi = lane + 64 * i;
+                    if (wi < nw) {
+                        const bool isd = wi >= W * NB;
+                        v[i] = ld_dev((isd ? dsrc : wsrc) + (isd ? wi - W * NB : wi));
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int wi = lane + 64 * i;
+                    if (wi >= nw) continue;
                     const bool isd = wi >= W * NB;
                     const int k = isd ? wi - W * NB : wi;
-                    const uint64_t *src = (isd ? dsrc : wsrc) + k;
-                    const uint64_t t0 = wall_clock64();
-                    uint64_t v = 0;
-                    for (uint32_t spin = 0;; ++spin) {
-                        v = ld_dev(src);
-                        if ((uint32_t)(v >> 32) == gs) break;
-                        if ((spin & 31) == 31 && ld_status(status) != 0) break;
-                        if (wall_clock64() - t0 > spin_ticks) {
-                            atomicCAS((int *)status, 0, (int)LA_ERR_TIMEOUT);
-                            break;
+                    if ((uint32_t)(v[i] >> 32) != gs) {
+                        const uint64_t *src = (isd ? dsrc : wsrc) + k;
+                        const uint64_t t0 = wall_clock64();
+                        for (uint32_t spin = 0;; ++spin) {
+                            v[i] = ld_dev(src);
+                            if ((uint32_t)(v[i] >> 32) == gs) break;
+                            if ((spin & 31) == 31 && ld_status(status) != 0) break;
+                            if (wall_clock64() - t0 > spin_ticks) {
+                                atomicCAS((int *)status, 0, (int)LA_ERR_TIMEOUT);
+                                break;
+                            }
                         }
                     }
-                    (isd ? s_drow : s_urow[j & 1])[k] = (uint32_t)v;
+                    (isd ? s_drow : s_urow[j & 1])[k] = (uint32_t)v[i];
                 }
             }
             if (lane == 0) {
