@@ -678,7 +678,11 @@ template <typename T>
 __global__ __launch_bounds__(256)
 void k_trsm_unit_lower(const T *Linv, int nb, T *B, int64_t ldb, int ncols)
 {
-    __shared__ T Li[64][65];
+    // rows of Li are 16-byte aligned so four consecutive t are one vector read
+    constexpr int LIS = 64 + 16 / (int)sizeof(T);
+    constexpr int VW = 16 / (int)sizeof(T);           // elements per 16-byte read
+    typedef T vec_t __attribute__((ext_vector_type(VW)));
+    __shared__ __align__(16) T Li[64][LIS];
     __shared__ T Bs[64][65];
     const int tid = threadIdx.x;
     const int c0 = blockIdx.x * 64;
@@ -696,11 +700,17 @@ void k_trsm_unit_lower(const T *Linv, int nb, T *B, int64_t ldb, int ncols)
     T acc[16];
 #pragma unroll
     for (int k = 0; k < 16; ++k) acc[k] = Bs[rq + 4 * k][cl];
-#pragma unroll 8
-    for (int t = 0; t < 64; ++t) {
-        const T b = Bs[t][cl];
+#pragma unroll 2
+    for (int t0 = 0; t0 < 64; t0 += VW) {
+        T b[VW];
 #pragma unroll
-        for (int k = 0; k < 16; ++k) acc[k] += Li[rq + 4 * k][t] * b;
+        for (int u = 0; u < VW; ++u) b[u] = Bs[t0 + u][cl];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const vec_t lv = *reinterpret_cast<const vec_t *>(&Li[rq + 4 * k][t0]);
+#pragma unroll
+            for (int u = 0; u < VW; ++u) acc[k] += lv[u] * b[u];
+        }
     }
     if (c0 + cl >= ncols) return;
 #pragma unroll

@@ -287,7 +287,11 @@ void k_trsv(int n, const TF *__restrict__ Tm, int64_t ldt, int nrhs, double *X, 
         const int a = e / nrhs, r = e - a * nrhs;
         double s = 0.0;
         if (a < rb) {
-            for (int c = 0; c < rb; ++c) s = __builtin_fma(Xi[a][c], ys[c][r], s);
+            // fixed trip count (Xi and ys are zero past rb, each such term
+            // leaves s unchanged) so the LDS reads are issued ahead of the
+            // FMA chain; the order of the sum is c ascending as before
+#pragma unroll 16
+            for (int c = 0; c < TB; ++c) s = __builtin_fma(Xi[a][c], ys[c][r], s);
             X[(int64_t)(r0 + a) * ldx + r] = s;
         }
         const uint64_t bits = (uint64_t)__double_as_longlong(s);
