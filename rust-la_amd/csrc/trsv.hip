@@ -58,9 +58,103 @@ __device__ void invert_diag_block(const double (*Ts)[TB + 1], int rb, double (*X
     }
 }
 
-template <typename TF, int MODE>
+// K factor tiles of blocks first .. first+count-1 (dependency order) for
+// this thread's row ir and sixteen columns q*16 .. q*16+15, into registers.
+// Every load is unconditional so that they issue back to back (a load under
+// a per-element branch waits for its data before the branch joins): the
+// column is clamped into the factor instead, and tiles past count repeat
+// the last one.  A clamped column meets a padding row of x, which is
+// published as zero.
+template <bool FWD, bool TRANS, int K, typename TF>
+__device__ inline void load_tiles(TF (&tv)[K][16], const TF *__restrict__ Tm, int64_t ldt, int n,
+                                  int nblocks, int r0, int ir, int q, int first, int count)
+{
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int step = first + min(k, count - 1);
+        const int c = FWD ? step : nblocks - 1 - step;
+        const int c0 = c * TB, cb = min(TB, n - c0);
+#pragma unroll
+        for (int cc = 0; cc < 16; ++cc) {
+            const int col = min(q * 16 + cc, cb - 1);
+            tv[k][cc] = !TRANS ? Tm[(int64_t)(r0 + ir) * ldt + c0 + col]
+                               : Tm[(int64_t)(c0 + col) * ldt + r0 + ir];
+        }
+    }
+}
+
+// Gather the tagged x words of blocks first .. first+count-1 into xs32:
+// sixteen independent loads in flight per thread, then validate; only
+// words that are not yet current are polled again.
+template <bool FWD>
+__device__ inline void gather_words(uint32_t *xs32, const uint64_t *xw, int words_per_block,
+                                    int nblocks, int first, int count, uint32_t epoch,
+                                    int32_t *err, uint64_t spin_ticks)
+{
+    const int tid = threadIdx.x;
+    const int nwords = count * words_per_block;
+    for (int e0 = tid; e0 < nwords; e0 += 16 * 256) {
+        uint64_t v[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int e = min(e0 + k * 256, nwords - 1);
+            const int kb = e / words_per_block, wi = e - kb * words_per_block;
+            const int step = first + kb;
+            const int c = FWD ? step : nblocks - 1 - step;
+            v[k] = ld_dev(xw + (size_t)c * words_per_block + wi);
+        }
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int e = e0 + k * 256;
+            if (e >= nwords) continue;
+            if ((uint32_t)(v[k] >> 32) != epoch) {
+                const int kb = e / words_per_block, wi = e - kb * words_per_block;
+                const int step = first + kb;
+                const int c = FWD ? step : nblocks - 1 - step;
+                const uint64_t *src = xw + (size_t)c * words_per_block + wi;
+                const uint64_t t0 = wall_clock64();
+                for (uint32_t spin = 0;; ++spin) {
+                    __builtin_amdgcn_s_sleep(1);
+                    v[k] = ld_dev(src);
+                    if ((uint32_t)(v[k] >> 32) == epoch) break;
+                    // a wait that failed anywhere ends every other wait quickly
+                    if ((spin & 63) == 63 && ld_dev(err) != 0) break;
+                    if (wall_clock64() - t0 > spin_ticks) {
+                        atomicCAS((int *)err, 0, (int)LA_ERR_TIMEOUT);
+                        break;
+                    }
+                }
+            }
+            xs32[e] = (uint32_t)v[k];
+        }
+    }
+}
+
+// acc[r] += sum over the first count tiles and this thread's sixteen
+// columns of tile * x, tiles and columns ascending.
+template <int NRB, int K, typename TF>
+__device__ inline void fold_tiles(double (&acc)[NRB], const TF (&tv)[K][16], const double *xs,
+                                  int nrhs, int q, int count)
+{
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        if (k < count) {
+            const double *xk = xs + (size_t)k * TB * nrhs;
+#pragma unroll
+            for (int cc = 0; cc < 16; ++cc) {
+                const int col = q * 16 + cc;
+#pragma unroll
+                for (int r = 0; r < NRB; ++r)
+                    if (r < nrhs)
+                        acc[r] = __builtin_fma((double)tv[k][cc], xk[col * nrhs + r], acc[r]);
+            }
+        }
+    }
+}
+
+template <typename TF, int MODE, int NRB>
 __global__ __launch_bounds__(256)
-void k_trsv(int n, const TF *__restrict__ Tm, int64_t ldt, int nrhs, double *X, int64_t ldx,
+void k_trsv(int n, const TF *__restrict__ Tm, int64_t ldt, int nrhs_arg, double *X, int64_t ldx,
             uint64_t *xw, int32_t *ticket, uint32_t epoch, int32_t *err, uint64_t spin_ticks)
 {
     // Single-block solves (n <= 64) keep the reference's update order with a
@@ -79,8 +173,11 @@ void k_trsv(int n, const TF *__restrict__ Tm, int64_t ldt, int nrhs, double *X, 
     double (*Ts)[TB + 1] = reinterpret_cast<double (*)[TB + 1]>(scratch);
     uint32_t *xs32 = reinterpret_cast<uint32_t *>(scratch);
     const double *xs = scratch;                              // [chunk][64][nrhs]
-    double (*red)[TB][NR] = reinterpret_cast<double (*)[TB][NR]>(scratch);  // [4][64][NR]
+    double (*red)[TB][NRB] = reinterpret_cast<double (*)[TB][NRB]>(scratch);  // [4][64][NRB]
 
+    // NRB = 1 compiles the single right-hand side without per-column
+    // guards; NRB = NR serves 2 .. NR columns
+    const int nrhs = (NRB == 1) ? 1 : nrhs_arg;
     const int tid = threadIdx.x;
     const int nblocks = (n + TB - 1) / TB;
     if (tid == 0) s_blk = atomicAdd(ticket, 1);
@@ -94,151 +191,68 @@ void k_trsv(int n, const TF *__restrict__ Tm, int64_t ldt, int nrhs, double *X, 
     // invert the diagonal block first: this overlaps the wait for the
     // blocks this one depends on
     if (!single) {
-        for (int e = tid; e < TB * TB; e += blockDim.x) {
-            const int a = e / TB, c = e - a * TB;
-            Ts[a][c] = (a < rb && c < rb) ? (double)Tm[(int64_t)(r0 + a) * ldt + r0 + c] : 0.0;
+        // sixteen unconditional (clamped) loads per thread, then the stores
+        TF dv[TB * TB / 256];
+#pragma unroll
+        for (int k = 0; k < TB * TB / 256; ++k) {
+            const int e = tid + 256 * k, a = e / TB, c = e - a * TB;
+            dv[k] = Tm[(int64_t)(r0 + min(a, rb - 1)) * ldt + r0 + min(c, rb - 1)];
+        }
+#pragma unroll
+        for (int k = 0; k < TB * TB / 256; ++k) {
+            const int e = tid + 256 * k, a = e / TB, c = e - a * TB;
+            Ts[a][c] = (a < rb && c < rb) ? (double)dv[k] : 0.0;
         }
         __syncthreads();
         invert_diag_block<MODE>(Ts, rb, Xi, part);
         __syncthreads();
     }
 
-    double acc[NR];
+    double acc[NRB];
 #pragma unroll
-    for (int r = 0; r < NR; ++r) acc[r] = 0.0;
+    for (int r = 0; r < NRB; ++r) acc[r] = 0.0;
     const int i = TRANS ? (tid & 63) : (tid >> 2);
     const int q = TRANS ? (tid >> 6) : (tid & 3);
-    // The last two dependencies (blocks t-2, t-1) form the final chunk, on
-    // the critical path: their 64 x 64 tiles of the factor and this block's
-    // slice of the right-hand side do not depend on x, so they are loaded
-    // now, before any wait.
-    const int nlast = min(t, 2);
-    TF tpre[2][16];
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-#pragma unroll
-        for (int cc = 0; cc < 16; ++cc) tpre[k][cc] = TF(0);
-        if (k < nlast && i < rb) {
-            const int step = t - nlast + k;
-            const int c = FWD ? step : nblocks - 1 - step;
-            const int c0 = c * TB, cb = min(TB, n - c0);
-#pragma unroll
-            for (int cc = 0; cc < 16; ++cc) {
-                const int col = q * 16 + cc;
-                if (col < cb)
-                    tpre[k][cc] = !TRANS ? Tm[(int64_t)(r0 + i) * ldt + c0 + col]
-                                         : Tm[(int64_t)(c0 + col) * ldt + r0 + i];
-            }
-        }
-    }
+    // rows past rb load clamped rows and are never stored
+    const int ir = min(i, rb - 1);
     double bpre = 0.0;
     const int ba = tid / max(nrhs, 1), br = tid - ba * max(nrhs, 1);
     if (tid < TB * nrhs && ba < rb) bpre = X[(int64_t)(r0 + ba) * ldx + br];
-    // Blocks per chunk: small, so that a workgroup never sits on a large
-    // batch of finished blocks while its immediate predecessor completes
-    // (the batch right before the last dependency is on the critical path).
-    const int CH = max(1, min(8, 64 / max(nrhs, 1)));
+    // The last two dependencies (blocks t-2, t-1) form the final chunk, on
+    // the critical path: their tiles do not depend on x and are loaded now,
+    // before any wait.
+    const int nlast = min(t, 2);
+    TF tpre[2][16];
+    load_tiles<FWD, TRANS>(tpre, Tm, ldt, n, nblocks, r0, ir, q, max(t - nlast, 0), max(nlast, 1));
+    // The other dependencies go in chunks whose factor tiles are loaded
+    // together before its x words are polled; small chunks keep a
+    // workgroup from sitting on a large batch of finished blocks while its
+    // immediate predecessor completes.
+    constexpr int CHT = sizeof(TF) == 8 ? 4 : 8;
+    const int CH = max(1, min(CHT, 64 / max(nrhs, 1)));
     const int words_per_block = TB * nrhs * 2;
 
-    // fold in finished blocks: all but the last two dependencies in chunks
-    // of CH, then the last two together (the final chunk, critical path)
     int done = 0;
-    while (done < t) {
-        const int chunk = (done < t - nlast) ? min(CH, t - nlast - done) : nlast;
-        // gather the tagged x words of blocks done .. done+chunk-1: sixteen
-        // independent loads in flight per thread, then validate; only words
-        // that are not yet current are polled again
-        const int nwords = chunk * words_per_block;
-        for (int e0 = tid; e0 < nwords; e0 += 16 * 256) {
-            uint64_t v[16];
-#pragma unroll
-            for (int k = 0; k < 16; ++k) {
-                const int e = e0 + k * 256;
-                v[k] = 0;
-                if (e < nwords) {
-                    const int kb = e / words_per_block, wi = e - kb * words_per_block;
-                    const int step = done + kb;
-                    const int c = FWD ? step : nblocks - 1 - step;
-                    v[k] = ld_dev(xw + (size_t)c * words_per_block + wi);
-                }
-            }
-#pragma unroll
-            for (int k = 0; k < 16; ++k) {
-                const int e = e0 + k * 256;
-                if (e >= nwords) continue;
-                if ((uint32_t)(v[k] >> 32) != epoch) {
-                    const int kb = e / words_per_block, wi = e - kb * words_per_block;
-                    const int step = done + kb;
-                    const int c = FWD ? step : nblocks - 1 - step;
-                    const uint64_t *src = xw + (size_t)c * words_per_block + wi;
-                    const uint64_t t0 = wall_clock64();
-                    for (uint32_t spin = 0;; ++spin) {
-                        __builtin_amdgcn_s_sleep(1);
-                        v[k] = ld_dev(src);
-                        if ((uint32_t)(v[k] >> 32) == epoch) break;
-                        // a wait that failed anywhere ends every other wait quickly
-                        if ((spin & 63) == 63 && ld_dev(err) != 0) break;
-                        if (wall_clock64() - t0 > spin_ticks) {
-                            atomicCAS((int *)err, 0, (int)LA_ERR_TIMEOUT);
-                            break;
-                        }
-                    }
-                }
-                xs32[e] = (uint32_t)v[k];
-            }
-        }
+    while (done < t - nlast) {
+        const int chunk = min(CH, t - nlast - done);
+        TF tv[CHT][16];
+        load_tiles<FWD, TRANS>(tv, Tm, ldt, n, nblocks, r0, ir, q, done, chunk);
+        gather_words<FWD>(xs32, xw, words_per_block, nblocks, done, chunk, epoch, err, spin_ticks);
         __syncthreads();
-        if (i < rb) {
-            if (done == t - nlast) {
-                // the final chunk: its tiles are already in registers
-#pragma unroll
-                for (int k = 0; k < 2; ++k) {
-                    if (k < nlast) {
-                        const double *xk = xs + (size_t)k * TB * nrhs;
-#pragma unroll
-                        for (int cc = 0; cc < 16; ++cc) {
-                            const int col = q * 16 + cc;
-#pragma unroll
-                            for (int r = 0; r < NR; ++r)
-                                if (r < nrhs)
-                                    acc[r] = __builtin_fma((double)tpre[k][cc],
-                                                           xk[col * nrhs + r], acc[r]);
-                        }
-                    }
-                }
-            } else {
-                for (int k = 0; k < chunk; ++k) {
-                    const int step = done + k;
-                    const int c = FWD ? step : nblocks - 1 - step;
-                    const int c0 = c * TB;
-                    const int cb = min(TB, n - c0);
-                    const double *xk = xs + (size_t)k * TB * nrhs;
-                    double tv[16];
-#pragma unroll
-                    for (int cc = 0; cc < 16; ++cc) {
-                        const int col = q * 16 + cc;
-                        tv[cc] = 0.0;
-                        if (col < cb)
-                            tv[cc] = !TRANS ? (double)Tm[(int64_t)(r0 + i) * ldt + c0 + col]
-                                            : (double)Tm[(int64_t)(c0 + col) * ldt + r0 + i];
-                    }
-#pragma unroll
-                    for (int cc = 0; cc < 16; ++cc) {
-                        const int col = q * 16 + cc;
-#pragma unroll
-                        for (int r = 0; r < NR; ++r)
-                            if (r < nrhs) acc[r] = __builtin_fma(tv[cc], xk[col * nrhs + r], acc[r]);
-                    }
-                }
-            }
-        }
+        fold_tiles<NRB>(acc, tv, xs, nrhs, q, chunk);
         __syncthreads();
         done += chunk;
+    }
+    if (nlast > 0) {
+        gather_words<FWD>(xs32, xw, words_per_block, nblocks, done, nlast, epoch, err, spin_ticks);
+        __syncthreads();
+        fold_tiles<NRB>(acc, tpre, xs, nrhs, q, nlast);
+        __syncthreads();
     }
 
     // reduce partial sums over the four column groups
 #pragma unroll
-    for (int r = 0; r < NR; ++r) red[q][i][r] = acc[r];
+    for (int r = 0; r < NRB; ++r) red[q][i][r] = acc[r];
     __syncthreads();
 
     const int lane = tid & 63, wv = tid >> 6;
@@ -269,7 +283,7 @@ void k_trsv(int n, const TF *__restrict__ Tm, int64_t ldt, int nrhs, double *X, 
         return;
     }
     // y = b - partial sums (into part-like scratch: reuse red[0])
-    __shared__ double ys[TB][NR];
+    __shared__ double ys[TB][NRB];
     for (int e = tid; e < TB * nrhs; e += blockDim.x) {
         const int a = e / nrhs, r = e - a * nrhs;
         double y = 0.0;
@@ -346,8 +360,12 @@ static int launch_trsv_mode(la_ctx *ctx, int64_t n, const TF *Tm, int64_t ldt, i
         uint32_t epoch = ++ctx->trsv_epoch;
         if (epoch == 0) epoch = ++ctx->trsv_epoch;
         if ((st = fill_i32(ctx, ticket, 1, 0))) return st;
-        hipLaunchKernelGGL((k_trsv<TF, MODE>), dim3(nblocks), dim3(256), 0, ctx->stream,
-                           (int)n, Tm, ldt, nr, X + rr, ldx, xw, ticket, epoch, d_err, ticks);
+        if (nr == 1)
+            hipLaunchKernelGGL((k_trsv<TF, MODE, 1>), dim3(nblocks), dim3(256), 0, ctx->stream,
+                               (int)n, Tm, ldt, nr, X + rr, ldx, xw, ticket, epoch, d_err, ticks);
+        else
+            hipLaunchKernelGGL((k_trsv<TF, MODE, NR>), dim3(nblocks), dim3(256), 0, ctx->stream,
+                               (int)n, Tm, ldt, nr, X + rr, ldx, xw, ticket, epoch, d_err, ticks);
         LA_HIP_CHECK(ctx, hipGetLastError());
     }
     return LA_OK;
