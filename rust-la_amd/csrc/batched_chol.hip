@@ -126,14 +126,24 @@ __device__ void chol_solve_one(int n, const double *__restrict__ Ap, double *Lp,
             for (int i = 0; i < CB_RTW; ++i) {
                 const int rt = wave + 4 * (CB_RTW * pass + i);
                 if (rt < rt_n) {
+                    // the element and its mirror, loaded unconditionally from
+                    // clamped (in-bounds) positions so that the eight loads
+                    // issue together; out-of-range lanes discard them
+                    double v[4], w[4];
+                    const int gcc = J0 + min(l16, jb - 1);
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int grc = J0 + min(16 * rt + lhi + 4 * r, m - 1);
+                        v[r] = Ap[(int64_t)grc * n + gcc];
+                        w[r] = Ap[(int64_t)gcc * n + grc];
+                    }
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
                         const int pr = 16 * rt + lhi + 4 * r;      // f64 MFMA row layout
                         if (pr < m && l16 < jb) {
                             const int gr = J0 + pr, gc = J0 + l16;
-                            const double v = Ap[(int64_t)gr * n + gc];
-                            if (gr > gc && !(v == Ap[(int64_t)gc * n + gr])) asym = 1;   // NaN too
-                            S.P[pr][l16] = v - acc[i][r];
+                            if (gr > gc && !(v[r] == w[r])) asym = 1;   // NaN too
+                            S.P[pr][l16] = v[r] - acc[i][r];
                         }
                     }
                 }

@@ -334,20 +334,27 @@ __device__ void bl_getrf(int n, typename BlCfg<MODE>::T *__restrict__ W, int *__
             for (int rt = wave; rt < rt_n; rt += 4) {
                 const int row = c_begin + 16 * rt + l16;
                 const bool rok = row < n;
-                const T *lrow = W + (int64_t)row * n + k0;
+                // unconditional loads from clamped positions (issued
+                // together); out-of-range elements are replaced by zero
+                const T *lrow = W + (int64_t)min(row, n - 1) * n + k0;
                 h8_t ah;
                 T af[NB / 4];
                 if constexpr (MODE == LA_PREC_FP16) {
+                    T lv[8];
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) lv[e] = lrow[min(8 * lhi + e, jb - 1)];
 #pragma unroll
                     for (int e = 0; e < 8; ++e) {
                         const int k = 8 * lhi + e;
-                        ah[e] = (_Float16)((rok && k < jb) ? lrow[k] : 0.0f);
+                        ah[e] = (_Float16)((rok && k < jb) ? lv[e] : 0.0f);
                     }
                 } else {
 #pragma unroll
+                    for (int s = 0; s < NB / 4; ++s) af[s] = lrow[min(4 * s + lhi, jb - 1)];
+#pragma unroll
                     for (int s = 0; s < NB / 4; ++s) {
                         const int k = 4 * s + lhi;
-                        af[s] = (rok && k < jb) ? lrow[k] : (T)0;
+                        if (!(rok && k < jb)) af[s] = (T)0;
                     }
                 }
                 for (int ct0 = 0; ct0 < ct_n; ct0 += G) {
@@ -533,7 +540,7 @@ __device__ void bl_residual(int n, int nrhs, int c0, int g, bool first,
         double acc[BL_MAXRHS];
 #pragma unroll
         for (int c = 0; c < BL_MAXRHS; ++c) acc[c] = 0.0;
-        const int row = i < n ? perm[i] : 0;
+        const int row = perm[min(i, n - 1)];       // unconditional; unused past n
         if (i < n && !first) {
             const double *ar = Ap + (int64_t)row * n;
             // 8 independent row loads in flight per lane
@@ -553,12 +560,16 @@ __device__ void bl_residual(int n, int nrhs, int c0, int g, bool first,
                     for (int msk = 8; msk >= 1; msk >>= 1) acc[c] += __shfl_xor(acc[c], msk);
                 }
         }
+        // the right-hand side row, loaded unconditionally (clamped column)
+        const double *br = Bp + (int64_t)row * nrhs + c0;
+        double bv[BL_MAXRHS];
+#pragma unroll
+        for (int c = 0; c < BL_MAXRHS; ++c) bv[c] = br[min(c, g - 1)];
         if (i < n && l16 == 0) {
-            const double *br = Bp + (int64_t)row * nrhs + c0;
 #pragma unroll
             for (int c = 0; c < BL_MAXRHS; ++c)
                 if (c < g) {
-                    const double r = br[c] - acc[c];
+                    const double r = bv[c] - acc[c];
                     Y[i * g + c] = (T)r;
                     rmax[c] = fmax(rmax[c], fabs(r));
                 }
