@@ -197,9 +197,11 @@ def main():
     tol = 1e-15 * n
     torch.cuda.synchronize()
 
-    # ---------------- FP32-IR (the measured path)
-    def ir_step():
-        return ctx.gesv_ir_dev(A, b, X, la_hip.PREC_FP32, tol, 50)
+    # ---------------- FP32-IR (the measured path): butterfly-transformed FP32
+    # LU without pivoting, FP64 refinement on A (falls back to the pivoted
+    # factor if refinement does not converge; the report says which ran)
+    def ir_step(prec=la_hip.PREC_FP32 | la_hip.IR_RBT):
+        return ctx.gesv_ir_dev(A, b, X, prec, tol, 50)
 
     for _ in range(args.warmup):
         ir_step()
@@ -219,6 +221,23 @@ def main():
     rep = reps[-1]
     fwd_ir = ((X - xs).abs().max() / xs.abs().max()).item()
     eta_ir = ctx.backward_error_dev(A, X, b)
+
+    # ---------------- FP32-IR with the partial-pivoting FP32 factor
+    Xp = X.clone()
+    ir_step(la_hip.PREC_FP32)
+    torch.cuda.synchronize()
+    tpv = []
+    for _ in range(args.steps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        rep_pv = ir_step(la_hip.PREC_FP32)
+        e1.record()
+        torch.cuda.synchronize()
+        tpv.append(e0.elapsed_time(e1))
+    ms_pv = sum(tpv) / len(tpv)
+    eta_pv = ctx.backward_error_dev(A, X, b)
+    fwd_pv = ((X - xs).abs().max() / xs.abs().max()).item()
+    X.copy_(Xp)
 
     # ---------------- plain FP64 LU + solve on the same system
     W = torch.empty_like(A)
@@ -278,16 +297,26 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32-factor+fp64-refine",
         "data": "synthetic: seeded Householder-product A = U diag(sigma) V^T, normal b",
-        "config": {"workload": f"dense solve n={n}, kappa={kappa:g}, 1 RHS, FP32 LU + FP64 IR "
-                               f"to eta<=1e-15*n", "n": n, "kappa": kappa, "nrhs": 1},
+        "config": {"workload": f"dense solve n={n}, kappa={kappa:g}, 1 RHS, FP32 LU (random "
+                               f"butterfly, no pivoting) + FP64 IR to eta<=1e-15*n",
+                   "n": n, "kappa": kappa, "nrhs": 1},
         "fp32_ir": {"ms": ms_ir, "tflops": flop / (ms_ir * 1e-3) / 1e12,
+                    "factor": "random butterfly U^T A V, FP32 LU without pivoting",
+                    "method": ["pivoted", "butterfly", "butterfly then pivoted"][rep.method],
+                    "perturbed_pivots": rep.perturbed_pivots,
                     "iterations": rep.iterations, "backward_error": eta_ir,
                     "forward_error_vs_xstar": fwd_ir, "factor_ms": rep.factor_ms,
                     "refine_ms": rep.refine_ms, "status": rep.status},
+        "fp32_ir_pivoted": {"ms": ms_pv, "tflops": flop / (ms_pv * 1e-3) / 1e12,
+                            "factor": "FP32 LU with partial pivoting",
+                            "iterations": rep_pv.iterations, "backward_error": eta_pv,
+                            "forward_error_vs_xstar": fwd_pv, "factor_ms": rep_pv.factor_ms,
+                            "refine_ms": rep_pv.refine_ms, "status": rep_pv.status},
         "fp64_lu": {"ms": ms_64, "tflops": flop / (ms_64 * 1e-3) / 1e12,
                     "backward_error": eta_64, "forward_error_vs_xstar": fwd_64,
                     "frac_of_fp64_peak": flop / (ms_64 * 1e-3) / 1e12 / PEAKS["fp64"]},
         "speedup_fp32ir_vs_fp64": ms_64 / ms_ir,
+        "speedup_fp32ir_pivoted_vs_fp64": ms_64 / ms_pv,
         "roofline": roof,
         "kernel_breakdown_ms": {k: round(v[1], 3) for k, v in stats.items()},
         "wall_s_timed": wall,

@@ -177,17 +177,36 @@ int la_dpotrs_dev(la_ctx *ctx, int64_t n, const double *L, int64_t ldl,
  * corrections and a normwise backward-error check
  *   eta = ||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf)   (per RHS, max).
  * A is never modified. */
+/* Option bit OR-ed into la_dgesv_ir_dev's prec (LA_PREC_FP32 only): factor
+ * a randomised butterfly transform U^T A V of A (depth-2 random butterflies)
+ * without pivoting instead of P A with partial pivoting.  The factorisation
+ * then needs no pivot search across the column; refinement runs on A
+ * itself, and a refinement that does not converge is repeated with the
+ * partial-pivoting factor. */
+#define LA_IR_RBT 0x100
+
+/* la_ir_report.method */
+enum la_ir_method {
+    LA_IR_METHOD_PIVOTED = 0,          /* partial-pivoting low-precision factor  */
+    LA_IR_METHOD_RBT = 1,              /* butterfly transform, no pivoting       */
+    LA_IR_METHOD_RBT_THEN_PIVOTED = 2  /* butterfly attempt did not converge;
+                                          the result is the pivoted attempt's  */
+};
+
 typedef struct la_ir_report {
     int32_t status;          /* LA_OK, LA_NOT_CONVERGED, LA_SINGULAR ...      */
     int32_t iterations;      /* refinement steps taken                        */
     int32_t prec;            /* la_prec used for the factorisation            */
-    int32_t reserved;
+    int32_t method;          /* la_ir_method                                  */
     double  backward_error;  /* final eta (max over RHS columns)              */
     double  initial_backward_error; /* eta of the first low-precision solve   */
     double  anorm_inf;       /* ||A||_inf                                     */
     double  factor_ms;       /* GPU time of the low-precision factorisation   */
     double  refine_ms;       /* GPU time of the refinement loop               */
-    double  total_ms;        /* convert + factor + refine                     */
+    double  total_ms;        /* convert + factor + refine (all attempts)      */
+    int32_t perturbed_pivots;/* butterfly form: pivots raised to the static
+                                threshold 2^-20 max|U^T A V|                  */
+    int32_t reserved;
 } la_ir_report;
 
 int la_dgesv_ir_dev(la_ctx *ctx, int64_t n, int64_t nrhs,

@@ -925,6 +925,255 @@ int getrf_core(la_ctx *ctx, int64_t m, int64_t n, T *A, int64_t lda, int64_t *d_
     return LA_OK;
 }
 
+// ------------------------------------------------ LU without pivoting
+// For the randomised (butterfly-transformed) factor of the refinement
+// solver, whose pivots need not follow partial pivoting: the panel becomes
+// a one-wavefront factorisation of the diagonal block plus an independent
+// product per 64-row block, with no exchange between workgroups.
+
+// Diagonal block (nb x nb at A, nb <= NB_IN) factored in place without
+// pivoting, in FP64, by one workgroup (padding rows and columns act as an
+// identity block).  A pivot smaller than tau in magnitude is replaced by
+// +-tau (static pivoting; the refinement corrects the perturbation) and
+// counted in *npert.  Also writes linv = L^-1 (unit lower) and uinv = U^-1,
+// NB_IN x NB_IN, zero outside the nb x nb triangles, formed in FP64 from the
+// FP64 factors.
+//
+// Factorisation: lane r of wave g holds the sixteen elements [r][16g ..
+// 16g+15] in registers.  Per step j (rolled, one barrier; LDS buffers
+// double-buffered by step parity) the wave holding column j publishes its
+// sixteen columns, lane j of every wave publishes its row segment, and every
+// element takes a[k] -= l_r * u_c with l_r = 0 for rows r <= j: one FMA
+// and no select.  Elements left of the diagonal of rows below it become
+// don't-cares; the multipliers l_r and the pivots are logged in LDS and
+// rebuild L and U's diagonal at the end.  Inverses: wave 0 forms the
+// columns of L^-1 and wave 1 those of U^-1 at the same time, one column per
+// lane, by right-looking substitution over the transposed factors.
+__global__ __launch_bounds__(256)
+void k_diag_nopiv(float *A, int64_t lda, int nb, const float *tau_p, float *linv, float *uinv,
+                  int32_t *npert)
+{
+    typedef double d2_t __attribute__((ext_vector_type(2)));
+    constexpr int LS = NB_IN + 2;                        // 16-byte aligned rows
+    __shared__ __align__(16) double LT[NB_IN][LS];       // LT[t][i] = L[i][t] (i > t)
+    __shared__ __align__(16) double UT[NB_IN][LS];       // UT[t][i] = U[i][t] (i <= t)
+    __shared__ __align__(16) double prow[2][NB_IN];      // row j, published by lane j
+    __shared__ __align__(16) double cblk[2][NB_IN][18];  // wave gj's sixteen columns
+    __shared__ double piv[NB_IN], rpiv[NB_IN];
+    const int tid = threadIdx.x, r = tid & 63, g = tid >> 6;
+    const double tau = fmax((double)*tau_p, 1e-30);
+    double a[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k)
+        a[k] = (double)A[(int64_t)min(r, nb - 1) * lda + min(16 * g + k, nb - 1)];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const int c = 16 * g + k;
+        if (!(r < nb && c < nb)) a[k] = (r == c) ? 1.0 : 0.0;
+    }
+    for (int j = 0; j < NB_IN; ++j) {
+        const int buf = j & 1, gj = j >> 4, kj = j & 15;
+        if (g == gj) {
+#pragma unroll
+            for (int k = 0; k < 16; k += 2)
+                *reinterpret_cast<d2_t *>(&cblk[buf][r][k]) = (d2_t){a[k], a[k + 1]};
+        }
+        if (r == j) {
+#pragma unroll
+            for (int k = 0; k < 16; k += 2)
+                *reinterpret_cast<d2_t *>(&prow[buf][16 * g + k]) = (d2_t){a[k], a[k + 1]};
+        }
+        __syncthreads();
+        // every LDS read of the step is issued before the reciprocal
+        double p = cblk[buf][j][kj];
+        const double cr = cblk[buf][r][kj];
+        d2_t u[8];
+#pragma unroll
+        for (int v = 0; v < 8; ++v) u[v] = *reinterpret_cast<const d2_t *>(&prow[buf][16 * g + 2 * v]);
+        if (!(fabs(p) >= tau)) p = (p < 0.0) ? -tau : tau;
+        // reciprocal: hardware estimate refined by two Newton steps
+        double rp = __builtin_amdgcn_rcp(p);
+        rp = fma(rp, fma(-p, rp, 1.0), rp);
+        rp = fma(rp, fma(-p, rp, 1.0), rp);
+        const double l = (r > j) ? cr * rp : 0.0;
+        if (g == 0) {
+            LT[j][r] = l;
+            if (r == 0) { piv[j] = p; rpiv[j] = rp; }
+        }
+#pragma unroll
+        for (int k = 0; k < 16; ++k) a[k] = fma(-l, u[k >> 1][k & 1], a[k]);
+    }
+    __syncthreads();
+    int pert = 0;
+    // U: rows r <= c as updated; diagonal: the (possibly raised) pivots;
+    // L: the logged multipliers (LT already holds them)
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const int c = 16 * g + k;
+        const double v = (r < c) ? a[k] : (r == c ? piv[c] : LT[c][r]);
+        if (r == c && v != a[k]) ++pert;
+        if (r < nb && c < nb) A[(int64_t)r * lda + c] = (float)v;
+        UT[c][r] = (r <= c) ? v : 0.0;
+    }
+    if (pert) atomicAdd(npert, pert);
+    __syncthreads();
+    const int c = r;
+    if (g == 0) {
+        // column c of L^-1: s = e_c; for t ascending: x_t = s_t, s_i -= L[i][t] x_t
+        double sx[NB_IN];
+#pragma unroll
+        for (int i = 0; i < NB_IN; ++i) sx[i] = (i == c) ? 1.0 : 0.0;
+#pragma unroll
+        for (int t = 0; t < NB_IN - 1; ++t) {
+#pragma unroll
+            for (int i0 = ((t + 1) / 2) * 2; i0 < NB_IN; i0 += 2) {
+                const d2_t l = *reinterpret_cast<const d2_t *>(&LT[t][i0]);
+                if (i0 > t) sx[i0] = fma(-l[0], sx[t], sx[i0]);
+                sx[i0 + 1] = fma(-l[1], sx[t], sx[i0 + 1]);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < NB_IN; ++i)
+            linv[i * NB_IN + c] = (i < nb && c < nb) ? (float)sx[i] : 0.f;
+    } else if (g == 1) {
+        // column c of U^-1: s = e_c; for t descending: y_t = s_t / U[t][t],
+        // s_i -= U[i][t] y_t (i < t)
+        double sy[NB_IN];
+#pragma unroll
+        for (int i = 0; i < NB_IN; ++i) sy[i] = (i == c) ? 1.0 : 0.0;
+#pragma unroll
+        for (int t = NB_IN - 1; t >= 0; --t) {
+            sy[t] *= rpiv[t];
+#pragma unroll
+            for (int i0 = 0; i0 < t; i0 += 2) {
+                const d2_t u = *reinterpret_cast<const d2_t *>(&UT[t][i0]);
+                sy[i0] = fma(-u[0], sy[t], sy[i0]);
+                if (i0 + 1 < t) sy[i0 + 1] = fma(-u[1], sy[t], sy[i0 + 1]);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < NB_IN; ++i)
+            uinv[i * NB_IN + c] = (i < nb && c < nb) ? (float)sy[i] : 0.f;
+    }
+}
+
+// L21 = A21 U11^-1 in place for rows [0, rows) of the nb-column block at A:
+// one workgroup per 64 rows; the block and U^-1 staged in LDS, thread (row
+// r, column group g) forms sixteen columns.
+template <typename T>
+__global__ __launch_bounds__(256)
+void k_l21_nopiv(T *A, int64_t lda, int rows, int nb, const T *__restrict__ uinv)
+{
+    __shared__ T Ui[NB_IN][NB_IN + 1];
+    __shared__ T As[NB_IN][NB_IN + 1];
+    const int tid = threadIdx.x;
+    const int r0 = blockIdx.x * 64;
+    T av[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const int e = tid + 256 * k, i = e >> 6, t = e & 63;
+        av[k] = A[(int64_t)min(r0 + i, rows - 1) * lda + min(t, nb - 1)];
+        Ui[i][t] = uinv[i * NB_IN + t];
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const int e = tid + 256 * k, i = e >> 6, t = e & 63;
+        As[i][t] = (t < nb) ? av[k] : T(0);
+    }
+    __syncthreads();
+    const int r = tid >> 2, g = tid & 3;
+    T acc[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) acc[k] = T(0);
+#pragma unroll 4
+    for (int t = 0; t < NB_IN; ++t) {
+        const T a = As[r][t];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) acc[k] = fma(a, Ui[t][g * 16 + k], acc[k]);
+    }
+    if (r0 + r >= rows) return;
+#pragma unroll
+    for (int k = 0; k < 16; ++k)
+        if (g * 16 + k < nb) A[(int64_t)(r0 + r) * lda + g * 16 + k] = acc[k];
+}
+
+// In-place LU of the m x n matrix A without pivoting (right-looking, the
+// same outer/inner blocking and trailing updates as getrf_core).  *d_tau is
+// the static-pivot threshold; *d_npert counts perturbed pivots.
+template <typename T>
+int getrf_nopiv_core(la_ctx *ctx, int64_t m, int64_t n, T *A, int64_t lda, const float *d_tau,
+                     int32_t *d_npert, int upd_prec)
+{
+    static_assert(sizeof(T) == 4, "the unpivoted factor is FP32");
+    if (m > INT32_MAX / 2 || n > INT32_MAX / 2)
+        return set_error(ctx, LA_ERR_ARG, "getrf_nopiv: extent too large");
+    const int64_t kmax = m < n ? m : n;
+    constexpr int NLINV = NB_OUT / NB_IN;
+    const size_t inv_bytes = (size_t)(NLINV + 1) * NB_IN * NB_IN * sizeof(T);
+    void *base = nullptr;
+    int st = ws_get(ctx, WS_PANEL, inv_bytes, &base);
+    if (st) return st;
+    T *linv_ring = (T *)base;
+    T *uinv = linv_ring + (size_t)NLINV * NB_IN * NB_IN;
+    if ((st = fill_i32(ctx, d_npert, 1, 0))) return st;
+
+    for (int64_t K0 = 0; K0 < kmax; K0 += NB_OUT) {
+        const int64_t nbo = std::min<int64_t>(NB_OUT, kmax - K0);
+        for (int64_t k = K0; k < K0 + nbo; k += NB_IN) {
+            const int nbi = (int)std::min<int64_t>(NB_IN, K0 + nbo - k);
+            T *Ap = A + k * lda + k;
+            T *linv = linv_ring + (size_t)((k - K0) / NB_IN) * NB_IN * NB_IN;
+            {
+                ProfScope ps(ctx, LA_K_PANEL, (double)(m - k) * nbi * nbi,
+                             2.0 * (m - k) * nbi * sizeof(T));
+                hipLaunchKernelGGL(k_diag_nopiv, dim3(1), dim3(256), 0, ctx->stream, Ap, lda,
+                                   nbi, d_tau, linv, uinv, d_npert);
+                LA_HIP_CHECK(ctx, hipGetLastError());
+                const int64_t below = m - k - nbi;
+                if (below > 0) {
+                    hipLaunchKernelGGL(k_l21_nopiv<T>, dim3((unsigned)ceil_div(below, 64)),
+                                       dim3(256), 0, ctx->stream, Ap + (int64_t)nbi * lda, lda,
+                                       (int)below, nbi, (const T *)uinv);
+                    LA_HIP_CHECK(ctx, hipGetLastError());
+                }
+            }
+            const int64_t rest = K0 + nbo - (k + nbi);
+            if (rest > 0) {
+                if ((st = launch_trsm<T>(ctx, linv, nbi, A + k * lda + k + nbi, lda, rest)))
+                    return st;
+                if (m - k - nbi > 0)
+                    if ((st = trail_gemm<T>(ctx, upd_prec, m - k - nbi, rest, nbi,
+                                            A + (k + nbi) * lda + k, lda, A + k * lda + k + nbi,
+                                            lda, A + (k + nbi) * lda + k + nbi, lda)))
+                        return st;
+            }
+        }
+        const int64_t ncol = n - (K0 + nbo);
+        if (ncol > 0) {
+            T *B12 = A + K0 * lda + K0 + nbo;
+            for (int64_t kb = K0; kb < K0 + nbo; kb += NB_IN) {
+                const int nbi = (int)std::min<int64_t>(NB_IN, K0 + nbo - kb);
+                const T *linv_kb = linv_ring + (size_t)((kb - K0) / NB_IN) * NB_IN * NB_IN;
+                if ((st = launch_trsm<T>(ctx, linv_kb, nbi, A + kb * lda + K0 + nbo, lda, ncol)))
+                    return st;
+                const int64_t below = K0 + nbo - (kb + nbi);
+                if (below > 0)
+                    if ((st = trail_gemm<T>(ctx, upd_prec, below, ncol, nbi,
+                                            A + (kb + nbi) * lda + kb, lda,
+                                            A + kb * lda + K0 + nbo, lda,
+                                            A + (kb + nbi) * lda + K0 + nbo, lda)))
+                        return st;
+            }
+            if (K0 + nbo < m)
+                if ((st = trail_gemm<T>(ctx, upd_prec, m - K0 - nbo, ncol, nbo,
+                                        A + (K0 + nbo) * lda + K0, lda, B12, lda,
+                                        A + (K0 + nbo) * lda + K0 + nbo, lda)))
+                    return st;
+        }
+    }
+    return LA_OK;
+}
+
 int pivsign(la_ctx *ctx, const int64_t *d_ipiv, int64_t kmax, int32_t *d_info)
 {
     hipLaunchKernelGGL(k_pivsign, dim3(1), dim3(256), 0, ctx->stream, d_ipiv, kmax, d_info);
@@ -936,6 +1185,8 @@ template int getrf_core<double>(la_ctx *, int64_t, int64_t, double *, int64_t, i
                                 uint64_t *, int32_t *, int);
 template int getrf_core<float>(la_ctx *, int64_t, int64_t, float *, int64_t, int64_t *,
                                uint64_t *, int32_t *, int);
+template int getrf_nopiv_core<float>(la_ctx *, int64_t, int64_t, float *, int64_t, const float *,
+                                     int32_t *, int);
 
 }  // namespace lahip
 

@@ -9,6 +9,16 @@
 //         eta = ||r||inf / (||A||inf ||x||inf + ||b||inf)
 //         stop if eta <= tol;  d = U^-1 L^-1 P r;  x += d
 // FP64 is used only for the residual, the update and the check.
+//
+// With LA_IR_RBT the FP32 factor is of a randomised butterfly transform of
+// A instead, factored without pivoting (Parker's random butterflies, depth 2,
+// as in Baboulin, Dongarra, Herrmann and Tomov, "Accelerating linear system
+// solutions using randomization techniques", ACM TOMS 39(2), 2013):
+//   Ap = [A 0; 0 I] of order np = 4 ceil(n/4);   Ar = U^T Ap V
+//   Ar = L U (no pivoting, static pivots);  correction d = V U^-1 L^-1 U^T r
+// The refinement itself (residual, update, check) is unchanged and runs on
+// the original A; a refinement that does not converge falls back to the
+// partial-pivoting factor.
 #include "la_common.hpp"
 #include <cmath>
 #include <cstring>
@@ -147,6 +157,145 @@ __global__ void k_axpy_rows(int64_t n, int nrhs, const double *D, int64_t ldd, d
     X[i * ldx + r] += D[i * ldd + r];
 }
 
+// ---------------------------------------------- randomised butterflies
+// A butterfly of order m is (1/sqrt 2) [R S; R -S], R and S diagonal with
+// entries exp(u/10), u uniform in [-1/2, 1/2).  The depth-2 transform of
+// order np is W = diag(B1, B2) B0 (B0 of order np, B1 and B2 of order np/2);
+// its parameters for one side are g0 (B0: R in [0, h), S in [h, np)) and g1
+// (B1: R in [0, q), S in [q, h); B2: R in [h, h+q), S in [h+q, np)), with
+// h = np/2 and q = np/4.  The four indices i, i+q, h+i, h+q+i (i < q) are
+// closed under both levels.  g holds U's g0, g1 then V's g0, g1.
+__global__ void k_rbt_params(int64_t np, uint64_t seed, double *g)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= 4 * np) return;
+    const uint64_t z = mix64(seed + GOLDEN * (uint64_t)(e + 1));
+    const double u = (double)(z >> 11) * 0x1.0p-53;
+    g[e] = exp((u - 0.5) / 10.0);
+}
+
+// v <- W^T v on the index group (i, i+q, h+i, h+q+i)
+__device__ inline void rbt_t4(double v[4], const double *g0, const double *g1, int64_t i,
+                              int64_t q, int64_t h)
+{
+    const double z0 = g1[i] * (v[0] + v[1]);
+    const double z1 = g1[q + i] * (v[0] - v[1]);
+    const double z2 = g1[h + i] * (v[2] + v[3]);
+    const double z3 = g1[h + q + i] * (v[2] - v[3]);
+    v[0] = 0.5 * g0[i] * (z0 + z2);
+    v[1] = 0.5 * g0[q + i] * (z1 + z3);
+    v[2] = 0.5 * g0[h + i] * (z0 - z2);
+    v[3] = 0.5 * g0[h + q + i] * (z1 - z3);
+}
+
+// v <- W v on the index group (i, i+q, h+i, h+q+i)
+__device__ inline void rbt_f4(double v[4], const double *g0, const double *g1, int64_t i,
+                              int64_t q, int64_t h)
+{
+    const double u0 = g0[i] * v[0] + g0[h + i] * v[2];
+    const double u2 = g0[i] * v[0] - g0[h + i] * v[2];
+    const double u1 = g0[q + i] * v[1] + g0[h + q + i] * v[3];
+    const double u3 = g0[q + i] * v[1] - g0[h + q + i] * v[3];
+    v[0] = 0.5 * (g1[i] * u0 + g1[q + i] * u1);
+    v[1] = 0.5 * (g1[i] * u0 - g1[q + i] * u1);
+    v[2] = 0.5 * (g1[h + i] * u2 + g1[h + q + i] * u3);
+    v[3] = 0.5 * (g1[h + i] * u2 - g1[h + q + i] * u3);
+}
+
+// W32 = fl32(U^T Ap V), Ap = [A 0; 0 I]; *amax_bits = max |W32| (as bits).
+// Thread (row group i = blockIdx.y, column group j): a 4 x 4 tile of
+// indices (i, i+q, h+i, h+q+i) x (j, j+q, h+j, h+q+j); consecutive threads
+// take consecutive columns, so every load and store is coalesced.
+__global__ __launch_bounds__(256)
+void k_rbt_matrix(int64_t n, int64_t np, const double *__restrict__ A, int64_t lda,
+                  const double *__restrict__ g, float *W, int64_t ldw, unsigned *amax_bits)
+{
+    __shared__ float wmax[4];
+    const int64_t q = np / 4, h = np / 2;
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t i = blockIdx.y;
+    float amax = 0.f;
+    if (j < q) {
+        const int64_t ri[4] = {i, i + q, h + i, h + q + i};
+        const int64_t cj[4] = {j, j + q, h + j, h + q + j};
+        double v[4][4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                v[a][b] = A[min(ri[a], n - 1) * lda + min(cj[b], n - 1)];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                if (!(ri[a] < n && cj[b] < n)) v[a][b] = (ri[a] == cj[b]) ? 1.0 : 0.0;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) rbt_t4(v[a], g + 2 * np, g + 3 * np, j, q, h);
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            double t[4] = {v[0][b], v[1][b], v[2][b], v[3][b]};
+            rbt_t4(t, g, g + np, i, q, h);
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                const float f = (float)t[a];
+                W[ri[a] * ldw + cj[b]] = f;
+                amax = fmaxf(amax, fabsf(f));
+            }
+        }
+    }
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) amax = fmaxf(amax, __shfl_xor(amax, m));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = amax;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        atomicMax(amax_bits, __float_as_uint(fmaxf(fmaxf(wmax[0], wmax[1]),
+                                                   fmaxf(wmax[2], wmax[3]))));
+}
+
+// static-pivot threshold: 2^-20 max |W32| (at least the smallest normal)
+__global__ void k_rbt_tau(const unsigned *amax_bits, float *tau)
+{
+    if (threadIdx.x == 0) *tau = fmaxf(__uint_as_float(*amax_bits) * 0x1.0p-20f, 1.17549435e-38f);
+}
+
+// Y (np x nrhs, ld nrhs) = U^T [R; 0]
+__global__ void k_rbt_left_vec(int64_t n, int64_t np, int nrhs, const double *__restrict__ g,
+                               const double *__restrict__ R, int64_t ldr, double *Y)
+{
+    const int64_t q = np / 4, h = np / 2;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= q * nrhs) return;
+    const int64_t i = e / nrhs, r = e - i * nrhs;
+    const int64_t ri[4] = {i, i + q, h + i, h + q + i};
+    double v[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) v[a] = R[min(ri[a], n - 1) * ldr + r];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+        if (ri[a] >= n) v[a] = 0.0;
+    rbt_t4(v, g, g + np, i, q, h);
+#pragma unroll
+    for (int a = 0; a < 4; ++a) Y[ri[a] * nrhs + r] = v[a];
+}
+
+// X[0:n] = (V Y)[0:n]
+__global__ void k_rbt_right_vec(int64_t n, int64_t np, int nrhs, const double *__restrict__ g,
+                                const double *__restrict__ Y, double *X, int64_t ldx)
+{
+    const int64_t q = np / 4, h = np / 2;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= q * nrhs) return;
+    const int64_t i = e / nrhs, r = e - i * nrhs;
+    const int64_t ri[4] = {i, i + q, h + i, h + q + i};
+    double v[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) v[a] = Y[ri[a] * nrhs + r];
+    rbt_f4(v, g + 2 * np, g + 3 * np, i, q, h);
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+        if (ri[a] < n) X[ri[a] * ldx + r] = v[a];
+}
+
 __global__ void k_zero_f64(double *p, int n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0.0;
@@ -176,20 +325,38 @@ struct IrWork {
     int64_t *ipiv = nullptr;
     double *R = nullptr, *D = nullptr, *norms = nullptr;
     int32_t *status = nullptr;
+    // randomised-butterfly form
+    bool rbt = false;
+    int64_t np = 0;                      // padded order (multiple of 4)
+    double *g = nullptr;                 // butterfly parameters (4 np)
+    double *Y = nullptr;                 // transformed right-hand sides (np x nrhs)
+    unsigned *amax = nullptr;
+    float *tau = nullptr;
+    int32_t *npert = nullptr;
 };
 
 static int ir_alloc(la_ctx *ctx, int64_t n, int64_t nrhs, IrWork &w)
 {
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    const int64_t ld32 = (n + 63) & ~(int64_t)63;   // pad rows to 256 B
-    size_t b32 = al((size_t)n * ld32 * sizeof(float));
+    const int64_t np = (n + 3) & ~(int64_t)3;       // butterfly order
+    const int64_t ld32 = (np + 63) & ~(int64_t)63;  // pad rows to 256 B
+    size_t b32 = al((size_t)np * ld32 * sizeof(float));
     size_t bperm = al((size_t)n * 8), bipiv = al((size_t)n * 8);
     size_t brd = al((size_t)n * nrhs * 8);
     size_t bn = al(8 * 8), bst = al(4 * 4), bsc = al((size_t)n * 8);
+    size_t bg = al((size_t)4 * np * 8), by = al((size_t)np * nrhs * 8), bsm = al(64);
     void *base = nullptr;
-    int st = ws_get(ctx, WS_LP, b32 + bperm + bipiv + 2 * brd + bn + bst + 2 * bsc, &base);
+    int st = ws_get(ctx, WS_LP,
+                    b32 + bperm + bipiv + 2 * brd + bn + bst + 2 * bsc + bg + by + bsm, &base);
     if (st) return st;
     char *p = (char *)base;
+    w.np = np;
+    w.g = (double *)p; p += bg;
+    w.Y = (double *)p; p += by;
+    w.amax = (unsigned *)p;
+    w.tau = (float *)(p + 16);
+    w.npert = (int32_t *)(p + 32);
+    p += bsm;
     w.A32 = (float *)p; p += b32;
     w.ld32 = ld32;
     w.perm = (uint64_t *)p; p += bperm;
@@ -208,6 +375,22 @@ static int lp_solve(la_ctx *ctx, int64_t n, int64_t nrhs, const IrWork &w, const
                     int64_t ldr, double *out, int64_t ldo)
 {
     int st;
+    if (w.rbt) {                     // d = V U^-1 L^-1 U^T r
+        const int64_t q = w.np / 4;
+        hipLaunchKernelGGL(k_rbt_left_vec, dim3((unsigned)ceil_div(q * nrhs, 256)), dim3(256), 0,
+                           ctx->stream, n, w.np, (int)nrhs, w.g, rhs, ldr, w.Y);
+        LA_HIP_CHECK(ctx, hipGetLastError());
+        if ((st = trsv_blocked<float>(ctx, 'L', false, true, w.np, w.A32, w.ld32, nrhs, w.Y,
+                                      nrhs)))
+            return st;
+        if ((st = trsv_blocked<float>(ctx, 'U', false, false, w.np, w.A32, w.ld32, nrhs, w.Y,
+                                      nrhs)))
+            return st;
+        hipLaunchKernelGGL(k_rbt_right_vec, dim3((unsigned)ceil_div(q * nrhs, 256)), dim3(256), 0,
+                           ctx->stream, n, w.np, (int)nrhs, w.g, w.Y, out, ldo);
+        LA_HIP_CHECK(ctx, hipGetLastError());
+        return LA_OK;
+    }
     if ((st = gather_rows(ctx, n, nrhs, w.perm, rhs, ldr, out, ldo))) return st;
     if ((st = trsv_blocked<float>(ctx, 'L', false, true, n, w.A32, w.ld32, nrhs, out, ldo)))
         return st;
@@ -238,43 +421,47 @@ struct EvTimer {
 };
 }  // namespace
 
-extern "C" int la_dgesv_ir_dev(la_ctx *ctx, int64_t n, int64_t nrhs, const double *A,
-                               int64_t lda, const double *B, int64_t ldb, double *X,
-                               int64_t ldx, int prec, double tol, int max_iter,
-                               la_ir_report *report)
+// One factor-and-refine attempt (partial pivoting, or the butterfly form
+// when w.rbt); fills report and returns its status.
+static int ir_attempt(la_ctx *ctx, int64_t n, int64_t nrhs, const double *A, int64_t lda,
+                      const double *B, int64_t ldb, double *X, int64_t ldx, int prec,
+                      double tol, int max_iter, IrWork &w, la_ir_report *report)
 {
-    if (!ctx || !A || !B || !X || !report) return LA_ERR_ARG;
-    if (n <= 0 || nrhs <= 0 || nrhs > 16 || lda < n || ldb < nrhs || ldx < nrhs)
-        return LA_ERR_ARG;
-    if (prec != LA_PREC_FP32 && prec != LA_PREC_BF16 && prec != LA_PREC_FP16)
-        return set_error(ctx, LA_ERR_ARG, "dgesv_ir: precision not supported");
-    std::memset(report, 0, sizeof(*report));
-    report->prec = prec;
-    IrWork w;
-    int st = ir_alloc(ctx, n, nrhs, w);
-    if (st) return st;
+    int st;
     EvTimer ev;
-    w.prec = prec;
     LA_HIP_CHECK(ctx, hipEventRecord(ev.e[0], ctx->stream));
-    if (prec == LA_PREC_FP32) {
-        hipLaunchKernelGGL(k_convert_f32, dim3((unsigned)ceil_div(n * n, 1024)), dim3(256), 0,
-                           ctx->stream, n, n, A, lda, w.A32, w.ld32);
+    if (w.rbt) {
+        const int64_t q = w.np / 4;
+        hipLaunchKernelGGL(k_rbt_params, dim3((unsigned)ceil_div(4 * w.np, 256)), dim3(256), 0,
+                           ctx->stream, w.np, (uint64_t)0x5EED0F0B0773ull, w.g);
+        if ((st = fill_i32(ctx, (int32_t *)w.amax, 1, 0))) return st;
+        hipLaunchKernelGGL(k_rbt_matrix, dim3((unsigned)ceil_div(q, 256), (unsigned)q), dim3(256),
+                           0, ctx->stream, n, w.np, A, lda, w.g, w.A32, w.ld32, w.amax);
+        hipLaunchKernelGGL(k_rbt_tau, dim3(1), dim3(64), 0, ctx->stream, w.amax, w.tau);
+        LA_HIP_CHECK(ctx, hipGetLastError());
+        if ((st = fill_i32(ctx, w.status, 1, 0))) return st;
+        if ((st = getrf_nopiv_core<float>(ctx, w.np, w.np, w.A32, w.ld32, w.tau, w.npert, prec)))
+            return st;
     } else {
-        // per-column power-of-two scaling keeps the rounded operands in range
-        if ((st = fill_u64(ctx, (uint64_t *)w.colmax, n, 0))) return st;
-        hipLaunchKernelGGL(k_colmax, dim3((unsigned)ceil_div(n, 256), (unsigned)ceil_div(n, 256)),
-                           dim3(256), 0, ctx->stream, n, A, lda, w.colmax);
-        hipLaunchKernelGGL(k_colscale, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0,
-                           ctx->stream, n, w.colmax, w.colscale);
-        hipLaunchKernelGGL(k_convert_f32_scaled, dim3((unsigned)ceil_div(n * n, 256)), dim3(256),
-                           0, ctx->stream, n, A, lda, w.colscale, w.A32, w.ld32);
+        if (prec == LA_PREC_FP32) {
+            hipLaunchKernelGGL(k_convert_f32, dim3((unsigned)ceil_div(n * n, 1024)), dim3(256), 0,
+                               ctx->stream, n, n, A, lda, w.A32, w.ld32);
+        } else {
+            // per-column power-of-two scaling keeps the rounded operands in range
+            if ((st = fill_u64(ctx, (uint64_t *)w.colmax, n, 0))) return st;
+            hipLaunchKernelGGL(k_colmax,
+                               dim3((unsigned)ceil_div(n, 256), (unsigned)ceil_div(n, 256)),
+                               dim3(256), 0, ctx->stream, n, A, lda, w.colmax);
+            hipLaunchKernelGGL(k_colscale, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0,
+                               ctx->stream, n, w.colmax, w.colscale);
+            hipLaunchKernelGGL(k_convert_f32_scaled, dim3((unsigned)ceil_div(n * n, 256)),
+                               dim3(256), 0, ctx->stream, n, A, lda, w.colscale, w.A32, w.ld32);
+        }
+        LA_HIP_CHECK(ctx, hipGetLastError());
+        if ((st = getrf_core<float>(ctx, n, n, w.A32, w.ld32, w.ipiv, w.perm, w.status, prec)))
+            return st;
     }
-    LA_HIP_CHECK(ctx, hipGetLastError());
-    if ((st = getrf_core<float>(ctx, n, n, w.A32, w.ld32, w.ipiv, w.perm, w.status, prec)))
-        return st;
     LA_HIP_CHECK(ctx, hipEventRecord(ev.e[1], ctx->stream));
-    // singular low-precision factor: report and stop
-    // (exact zero on U's diagonal makes every correction infinite)
     if ((st = lp_solve(ctx, n, nrhs, w, B, ldb, X, ldx))) return st;
     double h[4] = {0, 0, 0, 0};
     int it = 0;
@@ -306,13 +493,53 @@ extern "C" int la_dgesv_ir_dev(la_ctx *ctx, int64_t n, int64_t nrhs, const doubl
     }
     report->iterations = it;
     report->backward_error = eta;
-    report->factor_ms = ev.ms(0, 1);
-    report->refine_ms = ev.ms(1, 2);
-    report->total_ms = ev.ms(0, 2);
+    report->factor_ms += ev.ms(0, 1);
+    report->refine_ms += ev.ms(1, 2);
+    report->total_ms += ev.ms(0, 2);
     if (fst != LA_OK) report->status = fst;
     else if (!(eta <= tol)) report->status = std::isfinite(eta) ? LA_NOT_CONVERGED : LA_SINGULAR;
     else report->status = LA_OK;
-    return report->status == LA_ERR_TIMEOUT ? LA_ERR_TIMEOUT : report->status;
+    return report->status;
+}
+
+extern "C" int la_dgesv_ir_dev(la_ctx *ctx, int64_t n, int64_t nrhs, const double *A,
+                               int64_t lda, const double *B, int64_t ldb, double *X,
+                               int64_t ldx, int prec, double tol, int max_iter,
+                               la_ir_report *report)
+{
+    if (!ctx || !A || !B || !X || !report) return LA_ERR_ARG;
+    if (n <= 0 || nrhs <= 0 || nrhs > 16 || lda < n || ldb < nrhs || ldx < nrhs)
+        return LA_ERR_ARG;
+    const bool rbt = (prec & LA_IR_RBT) != 0;
+    prec &= ~LA_IR_RBT;
+    if (prec != LA_PREC_FP32 && prec != LA_PREC_BF16 && prec != LA_PREC_FP16)
+        return set_error(ctx, LA_ERR_ARG, "dgesv_ir: precision not supported");
+    if (rbt && (prec != LA_PREC_FP32 || ((n + 3) / 4) > 65535))
+        return set_error(ctx, LA_ERR_ARG,
+                         "dgesv_ir: the butterfly form needs FP32 and n <= 262140");
+    std::memset(report, 0, sizeof(*report));
+    report->prec = prec;
+    IrWork w;
+    int st = ir_alloc(ctx, n, nrhs, w);
+    if (st) return st;
+    w.prec = prec;
+    w.rbt = rbt;
+    report->method = rbt ? LA_IR_METHOD_RBT : LA_IR_METHOD_PIVOTED;
+    st = ir_attempt(ctx, n, nrhs, A, lda, B, ldb, X, ldx, prec, tol, max_iter, w, report);
+    if (rbt && (st == LA_NOT_CONVERGED || st == LA_SINGULAR)) {
+        // the unpivoted factor did not serve: partial pivoting, same budget
+        int32_t np = 0;
+        LA_HIP_CHECK(ctx, hipMemcpy(&np, w.npert, 4, hipMemcpyDeviceToHost));
+        report->perturbed_pivots = np;
+        w.rbt = false;
+        report->method = LA_IR_METHOD_RBT_THEN_PIVOTED;
+        st = ir_attempt(ctx, n, nrhs, A, lda, B, ldb, X, ldx, prec, tol, max_iter, w, report);
+    } else if (rbt) {
+        int32_t np = 0;
+        LA_HIP_CHECK(ctx, hipMemcpy(&np, w.npert, 4, hipMemcpyDeviceToHost));
+        report->perturbed_pivots = np;
+    }
+    return st;
 }
 
 extern "C" int la_dgesv_ir_rm(la_ctx *ctx, int64_t n, int64_t nrhs, const double *A,

@@ -153,6 +153,10 @@ int gemm_rm(la_ctx *ctx, int64_t M, int64_t N, int64_t K, T alpha,
 template <typename T>
 int getrf_core(la_ctx *ctx, int64_t m, int64_t n, T *A, int64_t lda, int64_t *d_ipiv,
                uint64_t *d_perm, int32_t *d_status, int upd_prec = LA_PREC_FP32);
+// LU without pivoting (static pivots below *d_tau, counted in *d_npert)
+template <typename T>
+int getrf_nopiv_core(la_ctx *ctx, int64_t m, int64_t n, T *A, int64_t lda, const float *d_tau,
+                     int32_t *d_npert, int upd_prec = LA_PREC_FP32);
 int gemm_lp(la_ctx *ctx, int prec, int64_t M, int64_t N, int64_t K, float alpha, const float *A,
             int64_t lda, const float *B, int64_t ldb, float beta, float *C, int64_t ldc);
 // d_info[0] = pospivsign (1 = even number of swaps)

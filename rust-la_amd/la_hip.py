@@ -29,6 +29,8 @@ LA_ERR_ARG = 9
 LA_ERR_TIMEOUT = 10
 
 PREC_FP64, PREC_FP32, PREC_BF16, PREC_FP16, PREC_FP32X3 = 0, 1, 2, 3, 4
+IR_RBT = 0x100      # option bit for gesv_ir_dev: butterfly transform, no pivoting
+IR_METHOD_PIVOTED, IR_METHOD_RBT, IR_METHOD_RBT_THEN_PIVOTED = 0, 1, 2
 
 KERNEL_CLASSES = ["gemm64", "gemm32", "gemm16", "panel", "trsv", "resid", "trsm", "rowmove",
                   "batch"]
@@ -51,11 +53,12 @@ class LaError(RuntimeError):
 
 class IrReport(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int32), ("iterations", ctypes.c_int32),
-                ("prec", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("prec", ctypes.c_int32), ("method", ctypes.c_int32),
                 ("backward_error", ctypes.c_double),
                 ("initial_backward_error", ctypes.c_double),
                 ("anorm_inf", ctypes.c_double), ("factor_ms", ctypes.c_double),
-                ("refine_ms", ctypes.c_double), ("total_ms", ctypes.c_double)]
+                ("refine_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
+                ("perturbed_pivots", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 _lib = None
@@ -330,7 +333,8 @@ class Context:
         st = self.lib.la_dgesv_ir_dev(self.h, n, B.shape[1], _ptr(A), A.stride(0), _ptr(B),
                                       B.stride(0), _ptr(X), X.stride(0), prec, tol, max_iter,
                                       ctypes.byref(rep))
-        if st not in (LA_OK, LA_NOT_CONVERGED):
+        # non-convergence and a singular factor are outcomes carried by the report
+        if st not in (LA_OK, LA_NOT_CONVERGED, LA_SINGULAR):
             self._check(st, "dgesv_ir_dev")
         return rep
 
