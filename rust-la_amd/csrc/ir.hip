@@ -93,58 +93,107 @@ __device__ inline void atomic_max_nonneg(double *addr, double v) {
               (unsigned long long)__double_as_longlong(v));
 }
 
-// R = B - A X for nrhs <= 16 right-hand sides, one workgroup per row.
-// norms[0] = max |R|, norms[1] = max |X|, norms[2] = max |B|,
-// norms[3] = max_i sum_j |A_ij| (only when want_anorm).
+// R = B - A X for nrhs <= 16 right-hand sides.  Each wavefront takes
+// RES_RPW rows in turn, a workgroup 4 * RES_RPW rows; every lane keeps eight
+// independent loads of A (and of X) in flight.  NRB = 1 is the single
+// right-hand side without per-column guards.  norms[0] = max |R|,
+// norms[1] = max |X|, norms[2] = max |B|, norms[3] = max_i sum_j |A_ij|
+// (only when want_anorm): reduced over the workgroup first, so each
+// workgroup issues one atomic per norm.  Maxima are taken over the bit
+// patterns of the non-negative values, so a NaN (above +inf) propagates.
+constexpr int RES_RPW = 4;
+template <int NRB>
 __global__ __launch_bounds__(256)
-void k_residual(int64_t n, int nrhs, const double *__restrict__ A, int64_t lda,
+void k_residual(int64_t n, int nrhs_arg, const double *__restrict__ A, int64_t lda,
                 const double *__restrict__ X, int64_t ldx, const double *__restrict__ B,
                 int64_t ldb, double *R, int64_t ldr, double *norms, int want_anorm)
 {
-    constexpr int NRM = 16;
-    __shared__ double red[4][NRM + 1];
-    const int64_t i = blockIdx.x;
-    const double *arow = A + i * lda;
-    double acc[NRM];
-#pragma unroll
-    for (int r = 0; r < NRM; ++r) acc[r] = 0.0;
-    double asum = 0.0;
-    for (int64_t j = threadIdx.x; j < n; j += blockDim.x) {
-        const double a = arow[j];
-        asum += fabs(a);
-        const double *xr = X + j * ldx;
-#pragma unroll
-        for (int r = 0; r < NRM; ++r)
-            if (r < nrhs) acc[r] += a * xr[r];
-    }
-    // reduce
-#pragma unroll
-    for (int r = 0; r < NRM; ++r) {
-        double v = (r < nrhs) ? acc[r] : 0.0;
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-        acc[r] = v;
-    }
-    for (int off = 32; off > 0; off >>= 1) asum += __shfl_xor(asum, off);
+    __shared__ unsigned long long red[4][4];
+    const int nrhs = (NRB == 1) ? 1 : nrhs_arg;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    if (lane == 0) {
+    typedef unsigned long long u64;
+    auto bits = [](double v) { return (u64)__double_as_longlong(fabs(v)); };
+    u64 mr = 0, mx = 0, mb = 0, ma = 0;
+    for (int rr = 0; rr < RES_RPW; ++rr) {
+        const int64_t i = ((int64_t)blockIdx.x * 4 + wv) * RES_RPW + rr;
+        if (i >= n) break;
+        const double *arow = A + i * lda;
+        double acc[NRB];
 #pragma unroll
-        for (int r = 0; r < NRM; ++r) red[wv][r] = acc[r];
-        red[wv][NRM] = asum;
+        for (int r = 0; r < NRB; ++r) acc[r] = 0.0;
+        double asum = 0.0;
+        int64_t j = lane;
+        for (; j + 64 * 7 < n; j += 64 * 8) {
+            double a[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) a[u] = arow[j + 64 * u];
+            if (NRB == 1) {
+                double xv[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) xv[u] = X[(j + 64 * u) * ldx];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    asum += fabs(a[u]);
+                    acc[0] = fma(a[u], xv[u], acc[0]);
+                }
+            } else {
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    asum += fabs(a[u]);
+                    const double *xr = X + (j + 64 * u) * ldx;
+#pragma unroll
+                    for (int r = 0; r < NRB; ++r)
+                        if (r < nrhs) acc[r] = fma(a[u], xr[r], acc[r]);
+                }
+            }
+        }
+        for (; j < n; j += 64) {
+            const double a = arow[j];
+            asum += fabs(a);
+            const double *xr = X + j * ldx;
+#pragma unroll
+            for (int r = 0; r < NRB; ++r)
+                if (r < nrhs) acc[r] = fma(a, xr[r], acc[r]);
+        }
+#pragma unroll
+        for (int r = 0; r < NRB; ++r) {
+            double v = acc[r];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+            acc[r] = v;
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) asum += __shfl_xor(asum, off);
+        ma = max(ma, bits(asum));
+        if (lane < nrhs) {
+            double ax = acc[0];
+#pragma unroll
+            for (int r = 1; r < NRB; ++r) ax = (r == lane) ? acc[r] : ax;
+            const double b = B[i * ldb + lane];
+            const double res = b - ax;
+            R[i * ldr + lane] = res;
+            mr = max(mr, bits(res));
+            mx = max(mx, bits(X[i * ldx + lane]));
+            mb = max(mb, bits(b));
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        mr = max(mr, (u64)__shfl_xor((long long)mr, off));
+        mx = max(mx, (u64)__shfl_xor((long long)mx, off));
+        mb = max(mb, (u64)__shfl_xor((long long)mb, off));
+    }
+    if (lane == 0) {
+        red[wv][0] = mr;
+        red[wv][1] = mx;
+        red[wv][2] = mb;
+        red[wv][3] = ma;
     }
     __syncthreads();
-    if (threadIdx.x < nrhs) {
-        const int r = threadIdx.x;
-        const double ax = (red[0][r] + red[1][r]) + (red[2][r] + red[3][r]);
-        const double b = B[i * ldb + r];
-        const double res = b - ax;
-        R[i * ldr + r] = res;
-        atomic_max_nonneg(norms + 0, fabs(res));
-        atomic_max_nonneg(norms + 1, fabs(X[i * ldx + r]));
-        atomic_max_nonneg(norms + 2, fabs(b));
-    }
-    if (threadIdx.x == 0 && want_anorm) {
-        const double s = (red[0][NRM] + red[1][NRM]) + (red[2][NRM] + red[3][NRM]);
-        atomic_max_nonneg(norms + 3, s);
+    if (threadIdx.x < 4) {
+        const int q = threadIdx.x;
+        const u64 v = max(max(red[0][q], red[1][q]), max(red[2][q], red[3][q]));
+        if (q < 3 || want_anorm) atomicMax(reinterpret_cast<u64 *>(norms + q), v);
     }
 }
 
@@ -308,8 +357,13 @@ int residual(la_ctx *ctx, int64_t n, int64_t nrhs, const double *A, int64_t lda,
     if (nrhs > 16) return set_error(ctx, LA_ERR_ARG, "residual: nrhs > 16");
     hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(64), 0, ctx->stream, d_norms, want_anorm ? 4 : 3);
     ProfScope ps(ctx, LA_K_RESID, 2.0 * n * n * nrhs, 8.0 * ((double)n * n + 3.0 * n * nrhs));
-    hipLaunchKernelGGL(k_residual, dim3((unsigned)n), dim3(256), 0, ctx->stream, n, (int)nrhs, A,
-                       lda, X, ldx, B, ldb, R, ldr, d_norms, want_anorm ? 1 : 0);
+    const unsigned grid = (unsigned)ceil_div(n, (int64_t)4 * RES_RPW);
+    if (nrhs == 1)
+        hipLaunchKernelGGL(k_residual<1>, dim3(grid), dim3(256), 0, ctx->stream, n, (int)nrhs, A,
+                           lda, X, ldx, B, ldb, R, ldr, d_norms, want_anorm ? 1 : 0);
+    else
+        hipLaunchKernelGGL(k_residual<16>, dim3(grid), dim3(256), 0, ctx->stream, n, (int)nrhs,
+                           A, lda, X, ldx, B, ldb, R, ldr, d_norms, want_anorm ? 1 : 0);
     LA_HIP_CHECK(ctx, hipGetLastError());
     return LA_OK;
 }
