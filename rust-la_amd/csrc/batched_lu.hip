@@ -175,9 +175,13 @@ __device__ void bl_getrf(int n, typename BlCfg<MODE>::T *__restrict__ W, int *__
         for (int s = 0; s < BL_SLOTS; ++s) {
             const int r = tid + BL_NT * s;
             pos[s] = r < m ? r : BL_BIG;
+            // unconditional loads from clamped positions, then zeroed
+            const T *wr = Wp + (int64_t)min(r, m - 1) * n;
+#pragma unroll
+            for (int c = 0; c < NB; ++c) a[s][c] = wr[min(c, jb - 1)];
 #pragma unroll
             for (int c = 0; c < NB; ++c)
-                a[s][c] = (r < m && c < jb) ? Wp[(int64_t)r * n + c] : (T)0;
+                if (!(r < m && c < jb)) a[s][c] = (T)0;
         }
         if (tid == 0) L.nmv = 0;
 #pragma unroll 1
@@ -306,9 +310,12 @@ __device__ void bl_getrf(int n, typename BlCfg<MODE>::T *__restrict__ W, int *__
             {
                 const int col = cc0 + tid;
                 T x[NB];
+                const int colc = min(col, n - 1);
+#pragma unroll
+                for (int i = 0; i < NB; ++i) x[i] = W[(int64_t)(k0 + min(i, jb - 1)) * n + colc];
 #pragma unroll
                 for (int i = 0; i < NB; ++i)
-                    x[i] = (tid < cw && i < jb) ? W[(int64_t)(k0 + i) * n + col] : (T)0;
+                    if (!(tid < cw && i < jb)) x[i] = (T)0;
 #pragma unroll
                 for (int i = 1; i < NB; ++i) {
                     T acc = x[i];
@@ -467,9 +474,13 @@ __device__ void bl_trsv_lu(int n, const T *__restrict__ W, T *Y, int g)
         for (int c = wave; c < g; c += 4) {
             const int i = lane;
             T lrow[64];
-            const T *wr = W + (int64_t)(b0 + i) * n + b0;
+            // unconditional loads from clamped positions, then zeroed
+            const T *wr = W + (int64_t)(b0 + min(i, bs - 1)) * n + b0;
 #pragma unroll
-            for (int k = 0; k < 64; ++k) lrow[k] = (i < bs && k < i) ? wr[k] : (T)0;
+            for (int k = 0; k < 64; ++k) lrow[k] = wr[min(k, bs - 1)];
+#pragma unroll
+            for (int k = 0; k < 64; ++k)
+                if (!(i < bs && k < i)) lrow[k] = (T)0;
             T y = i < bs ? Y[(b0 + i) * g + c] : (T)0;
 #pragma unroll
             for (int k = 0; k < 63; ++k) {
@@ -493,9 +504,12 @@ __device__ void bl_trsv_lu(int n, const T *__restrict__ W, T *Y, int g)
         for (int c = wave; c < g; c += 4) {
             const int i = lane;
             T urow[64];
-            const T *wr = W + (int64_t)(b0 + i) * n + b0;
+            const T *wr = W + (int64_t)(b0 + min(i, bs - 1)) * n + b0;
 #pragma unroll
-            for (int k = 0; k < 64; ++k) urow[k] = (i < bs && k >= i && k < bs) ? wr[k] : (T)1;
+            for (int k = 0; k < 64; ++k) urow[k] = wr[min(k, bs - 1)];
+#pragma unroll
+            for (int k = 0; k < 64; ++k)
+                if (!(i < bs && k >= i && k < bs)) urow[k] = (T)1;
             T y = i < bs ? Y[(b0 + i) * g + c] : (T)0;
 #pragma unroll
             for (int k = 63; k >= 0; --k) {

@@ -93,39 +93,50 @@ void k_gemm_rm(int M, int N, int K, T alpha, const T *__restrict__ A, int64_t ld
     auto load_tile = [&](int kt) {
         const int k0 = kt * GBK;
         const bool full = full_mn && (k0 + GBK <= K);
+        // edge tiles load from clamped (in-bounds) positions unconditionally,
+        // so the eight loads issue together, and zero what lies outside
         {
             const int gr = m0 + ar;
-            const T *src = A + (int64_t)gr * lda + k0 + ak;
             if (full) {
+                const T *src = A + (int64_t)gr * lda + k0 + ak;
 #pragma unroll
                 for (int e = 0; e < 8; ++e) ra[e] = src[e];
             } else {
+                const T *src = A + (int64_t)min(gr, M - 1) * lda;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) ra[e] = src[min(k0 + ak + e, K - 1)];
 #pragma unroll
                 for (int e = 0; e < 8; ++e)
-                    ra[e] = (gr < M && k0 + ak + e < K) ? src[e] : T(0);
+                    if (!(gr < M && k0 + ak + e < K)) ra[e] = T(0);
             }
         }
         if (!TRANS_B) {
             const int gk = k0 + bk_r;
-            const T *src = B + (int64_t)gk * ldb + n0 + bc;
             if (full) {
+                const T *src = B + (int64_t)gk * ldb + n0 + bc;
 #pragma unroll
                 for (int e = 0; e < 8; ++e) rb[e] = src[e];
             } else {
+                const T *src = B + (int64_t)min(gk, K - 1) * ldb;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) rb[e] = src[min(n0 + bc + e, N - 1)];
 #pragma unroll
                 for (int e = 0; e < 8; ++e)
-                    rb[e] = (gk < K && n0 + bc + e < N) ? src[e] : T(0);
+                    if (!(gk < K && n0 + bc + e < N)) rb[e] = T(0);
             }
         } else {
             const int gn = n0 + bt_n;
-            const T *src = B + (int64_t)gn * ldb + k0 + bt_k;
             if (full) {
+                const T *src = B + (int64_t)gn * ldb + k0 + bt_k;
 #pragma unroll
                 for (int e = 0; e < 8; ++e) rb[e] = src[e];
             } else {
+                const T *src = B + (int64_t)min(gn, N - 1) * ldb;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) rb[e] = src[min(k0 + bt_k + e, K - 1)];
 #pragma unroll
                 for (int e = 0; e < 8; ++e)
-                    rb[e] = (gn < N && k0 + bt_k + e < K) ? src[e] : T(0);
+                    if (!(gn < N && k0 + bt_k + e < K)) rb[e] = T(0);
             }
         }
     };

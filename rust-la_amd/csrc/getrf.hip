@@ -688,13 +688,24 @@ void k_trsm_unit_lower(const T *Linv, int nb, T *B, int64_t ldb, int ncols)
     const int c0 = blockIdx.x * 64;
     const int cl = tid & 63, rq = tid >> 6;
     // staged with a zero diagonal: row i then sums B[i] + Linv[i][t] B[t]
-    // over t ascending, the t >= i terms adding exact zeros
-    for (int e = tid; e < 64 * 64; e += 256) {
-        const int i = e >> 6, t = e & 63;
-        Li[i][t] = (i < nb && t < nb && t != i) ? Linv[i * NB_IN + t] : T(0);
+    // over t ascending, the t >= i terms adding exact zeros.  All 32 loads
+    // per thread are unconditional (clamped positions) and issue together;
+    // out-of-range elements are replaced by zero.
+    T lv[16], bv[16];
+    const int cc = min(c0 + cl, ncols - 1);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const int e = tid + 256 * k, i = e >> 6, t = e & 63;
+        lv[k] = Linv[min(i, nb - 1) * NB_IN + min(t, nb - 1)];
+        bv[k] = B[(int64_t)min(rq + 4 * k, nb - 1) * ldb + cc];
     }
-    for (int i = rq; i < 64; i += 4)
-        Bs[i][cl] = (i < nb && c0 + cl < ncols) ? B[(int64_t)i * ldb + c0 + cl] : T(0);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const int e = tid + 256 * k, i = e >> 6, t = e & 63;
+        Li[i][t] = (i < nb && t < nb && t != i) ? lv[k] : T(0);
+        const int ib = rq + 4 * k;
+        Bs[ib][cl] = (ib < nb && c0 + cl < ncols) ? bv[k] : T(0);
+    }
     __syncthreads();
     // row i = rq + 4k: s = B[i] + sum_{t<i} Linv[i][t] B[t], t ascending
     T acc[16];
@@ -1108,7 +1119,8 @@ int getrf_nopiv_core(la_ctx *ctx, int64_t m, int64_t n, T *A, int64_t lda, const
     if (m > INT32_MAX / 2 || n > INT32_MAX / 2)
         return set_error(ctx, LA_ERR_ARG, "getrf_nopiv: extent too large");
     const int64_t kmax = m < n ? m : n;
-    constexpr int NLINV = NB_OUT / NB_IN;
+    constexpr int NBO = 2 * NB_OUT;               // outer block width (trailing GEMM K)
+    constexpr int NLINV = NBO / NB_IN;
     const size_t inv_bytes = (size_t)(NLINV + 1) * NB_IN * NB_IN * sizeof(T);
     void *base = nullptr;
     int st = ws_get(ctx, WS_PANEL, inv_bytes, &base);
@@ -1117,8 +1129,8 @@ int getrf_nopiv_core(la_ctx *ctx, int64_t m, int64_t n, T *A, int64_t lda, const
     T *uinv = linv_ring + (size_t)NLINV * NB_IN * NB_IN;
     if ((st = fill_i32(ctx, d_npert, 1, 0))) return st;
 
-    for (int64_t K0 = 0; K0 < kmax; K0 += NB_OUT) {
-        const int64_t nbo = std::min<int64_t>(NB_OUT, kmax - K0);
+    for (int64_t K0 = 0; K0 < kmax; K0 += NBO) {
+        const int64_t nbo = std::min<int64_t>(NBO, kmax - K0);
         for (int64_t k = K0; k < K0 + nbo; k += NB_IN) {
             const int nbi = (int)std::min<int64_t>(NB_IN, K0 + nbo - k);
             T *Ap = A + k * lda + k;

@@ -93,10 +93,20 @@ void k_trsm_right_lt(const double *Lm, int64_t ldl, int kb, double *B, int64_t l
     __shared__ double Bs[64][65];
     const int tid = threadIdx.x;
     const int row0 = blockIdx.x * 64;
-    for (int e = tid; e < 64 * 64; e += 256) {
-        int r = e >> 6, c = e & 63;
-        Ls[r][c] = (r < kb && c <= r) ? Lm[(int64_t)r * ldl + c] : 0.0;
-        Bs[r][c] = (row0 + r < nrows && c < kb) ? B[(int64_t)(row0 + r) * ldb + c] : 0.0;
+    // unconditional loads from clamped positions (issued together), then
+    // the out-of-range elements are zeroed
+    double lv[16], bv[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const int e = tid + 256 * k, r = e >> 6, c = e & 63;
+        lv[k] = Lm[(int64_t)min(r, kb - 1) * ldl + min(c, kb - 1)];
+        bv[k] = B[(int64_t)min(row0 + r, nrows - 1) * ldb + min(c, kb - 1)];
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const int e = tid + 256 * k, r = e >> 6, c = e & 63;
+        Ls[r][c] = (r < kb && c <= r) ? lv[k] : 0.0;
+        Bs[r][c] = (row0 + r < nrows && c < kb) ? bv[k] : 0.0;
     }
     __syncthreads();
     const int r = tid & 63, cq = tid >> 6;
