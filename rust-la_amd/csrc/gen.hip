@@ -34,15 +34,25 @@ __global__ __launch_bounds__(256) void k_gen_sqnorm(int64_t n, const double *V, 
     if (threadIdx.x == 0) out[blockIdx.x] = (s[0] + s[1]) + (s[2] + s[3]);
 }
 
-// w[j] += sum_{i in chunk} v[i] A[i][j]   (w = A^T v), grid (col blocks, row chunks)
+// part[y][j] = sum_{i in row chunk y} v[i] A[i][j], grid (col blocks, row chunks);
+// w = A^T v is then summed over the chunks in order (no atomics: the
+// generated matrix is the same bit for bit in every run)
 __global__ __launch_bounds__(256) void k_gen_gemv_t(int64_t n, const double *A, int64_t lda,
-                                                    const double *v, double *w) {
+                                                    const double *v, double *part) {
     const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (j >= n) return;
     const int64_t i0 = (int64_t)blockIdx.y * 256, i1 = min(n, i0 + 256);
     double acc = 0.0;
     for (int64_t i = i0; i < i1; ++i) acc += v[i] * A[i * lda + j];
-    atomicAdd(w + j, acc);
+    part[(int64_t)blockIdx.y * n + j] = acc;
+}
+
+__global__ void k_gen_sum_parts(int64_t n, int nparts, const double *part, double *w) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    double s = 0.0;
+    for (int y = 0; y < nparts; ++y) s += part[(int64_t)y * n + j];
+    w[j] = s;
 }
 
 // w[i] = sum_j A[i][j] v[j]   (w = A v), one workgroup per row
@@ -178,12 +188,15 @@ extern "C" int la_gen_svd_matrix_dev(la_ctx *ctx, int64_t n, double kappa, int n
     if (nrhs < 0 || nrhs > 16 || (nrhs > 0 && (!b || !xstar))) return LA_ERR_ARG;
     const size_t vbytes = (size_t)n * 8;
     void *base = nullptr;
-    int st = ws_get(ctx, WS_GEN, (size_t)2 * nrefl * vbytes + vbytes + 2 * nrefl * 8 + 1024, &base);
+    const int nparts = (int)ceil_div(n, 256);
+    int st = ws_get(ctx, WS_GEN,
+                    (size_t)(2 * nrefl + 1 + nparts) * vbytes + 2 * nrefl * 8 + 1024, &base);
     if (st) return st;
     double *U = (double *)base;
     double *V = U + (size_t)nrefl * n;
     double *w = V + (size_t)nrefl * n;
-    double *sq = w + n;    // [0, nrefl): |u_i|^2, [nrefl, 2 nrefl): |v_i|^2
+    double *part = w + n;  // nparts x n partial sums of A^T v
+    double *sq = part + (size_t)nparts * n;  // [0, nrefl): |u_i|^2, [nrefl, 2 nrefl): |v_i|^2
     for (int i = 0; i < nrefl; ++i) {
         if ((st = la_fill_normal_dev(ctx, seed, u_stream + i, n, U + (size_t)i * n))) return st;
         if ((st = la_fill_normal_dev(ctx, seed, v_stream + i, n, V + (size_t)i * n))) return st;
@@ -197,9 +210,10 @@ extern "C" int la_gen_svd_matrix_dev(la_ctx *ctx, int64_t n, double kappa, int n
     LA_HIP_CHECK(ctx, hipGetLastError());
     const dim3 gt((unsigned)ceil_div(n, 256), (unsigned)ceil_div(n, 256));
     for (int i = nrefl - 1; i >= 0; --i) {       // A <- H_i A
-        LA_HIP_CHECK(ctx, hipMemsetAsync(w, 0, vbytes, ctx->stream));
         hipLaunchKernelGGL(k_gen_gemv_t, gt, dim3(256), 0, ctx->stream, n, A, lda,
-                           U + (size_t)i * n, w);
+                           U + (size_t)i * n, part);
+        hipLaunchKernelGGL(k_gen_sum_parts, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0,
+                           ctx->stream, n, nparts, part, w);
         hipLaunchKernelGGL(k_gen_rank1, dim3(g2), dim3(256), 0, ctx->stream, n, A, lda,
                            U + (size_t)i * n, w, sq + i);
     }

@@ -951,15 +951,17 @@ int getrf_core(la_ctx *ctx, int64_t m, int64_t n, T *A, int64_t lda, int64_t *d_
 // FP64 factors.
 //
 // Factorisation: lane r of wave g holds the sixteen elements [r][16g ..
-// 16g+15] in registers.  Per step j (rolled, one barrier; LDS buffers
-// double-buffered by step parity) the wave holding column j publishes its
-// sixteen columns, lane j of every wave publishes its row segment, and every
+// 16g+15] in registers.  Per step j (one barrier; LDS buffers
+// double-buffered by step parity; unrolled over the sixteen columns of a
+// register block so that column j's register index is static) the wave
+// holding column j publishes it, lane j of every wave publishes its row
+// segment, and every
 // element takes a[k] -= l_r * u_c with l_r = 0 for rows r <= j: one FMA
 // and no select.  Elements left of the diagonal of rows below it become
 // don't-cares; the multipliers l_r and the pivots are logged in LDS and
-// rebuild L and U's diagonal at the end.  Inverses: wave 0 forms the
-// columns of L^-1 and wave 1 those of U^-1 at the same time, one column per
-// lane, by right-looking substitution over the transposed factors.
+// rebuild L and U's diagonal at the end.  Inverses: 2 x 2 blocked, the
+// four 32 x 32 diagonal-block inverses one per wave, then two small
+// products for the off-diagonal blocks.
 __global__ __launch_bounds__(256)
 void k_diag_nopiv(float *A, int64_t lda, int nb, const float *tau_p, float *linv, float *uinv,
                   int32_t *npert)
@@ -969,7 +971,7 @@ void k_diag_nopiv(float *A, int64_t lda, int nb, const float *tau_p, float *linv
     __shared__ __align__(16) double LT[NB_IN][LS];       // LT[t][i] = L[i][t] (i > t)
     __shared__ __align__(16) double UT[NB_IN][LS];       // UT[t][i] = U[i][t] (i <= t)
     __shared__ __align__(16) double prow[2][NB_IN];      // row j, published by lane j
-    __shared__ __align__(16) double cblk[2][NB_IN][18];  // wave gj's sixteen columns
+    __shared__ double pcol[2][NB_IN];                    // column j, published by wave gj
     __shared__ double piv[NB_IN], rpiv[NB_IN];
     const int tid = threadIdx.x, r = tid & 63, g = tid >> 6;
     const double tau = fmax((double)*tau_p, 1e-30);
@@ -982,37 +984,37 @@ void k_diag_nopiv(float *A, int64_t lda, int nb, const float *tau_p, float *linv
         const int c = 16 * g + k;
         if (!(r < nb && c < nb)) a[k] = (r == c) ? 1.0 : 0.0;
     }
-    for (int j = 0; j < NB_IN; ++j) {
-        const int buf = j & 1, gj = j >> 4, kj = j & 15;
-        if (g == gj) {
+    for (int gj = 0; gj < NB_IN / 16; ++gj) {
 #pragma unroll
-            for (int k = 0; k < 16; k += 2)
-                *reinterpret_cast<d2_t *>(&cblk[buf][r][k]) = (d2_t){a[k], a[k + 1]};
+        for (int kj = 0; kj < 16; ++kj) {
+            const int j = 16 * gj + kj, buf = kj & 1;
+            if (g == gj) pcol[buf][r] = a[kj];
+            if (r == j) {
+#pragma unroll
+                for (int k = 0; k < 16; k += 2)
+                    *reinterpret_cast<d2_t *>(&prow[buf][16 * g + k]) = (d2_t){a[k], a[k + 1]};
+            }
+            __syncthreads();
+            // every LDS read of the step is issued before the reciprocal
+            double p = pcol[buf][j];
+            const double cr = pcol[buf][r];
+            d2_t u[8];
+#pragma unroll
+            for (int v = 0; v < 8; ++v)
+                u[v] = *reinterpret_cast<const d2_t *>(&prow[buf][16 * g + 2 * v]);
+            if (!(fabs(p) >= tau)) p = (p < 0.0) ? -tau : tau;
+            // reciprocal: hardware estimate refined by two Newton steps
+            double rp = __builtin_amdgcn_rcp(p);
+            rp = fma(rp, fma(-p, rp, 1.0), rp);
+            rp = fma(rp, fma(-p, rp, 1.0), rp);
+            const double l = (r > j) ? cr * rp : 0.0;
+            if (g == 0) {
+                LT[j][r] = l;
+                if (r == 0) { piv[j] = p; rpiv[j] = rp; }
+            }
+#pragma unroll
+            for (int k = 0; k < 16; ++k) a[k] = fma(-l, u[k >> 1][k & 1], a[k]);
         }
-        if (r == j) {
-#pragma unroll
-            for (int k = 0; k < 16; k += 2)
-                *reinterpret_cast<d2_t *>(&prow[buf][16 * g + k]) = (d2_t){a[k], a[k + 1]};
-        }
-        __syncthreads();
-        // every LDS read of the step is issued before the reciprocal
-        double p = cblk[buf][j][kj];
-        const double cr = cblk[buf][r][kj];
-        d2_t u[8];
-#pragma unroll
-        for (int v = 0; v < 8; ++v) u[v] = *reinterpret_cast<const d2_t *>(&prow[buf][16 * g + 2 * v]);
-        if (!(fabs(p) >= tau)) p = (p < 0.0) ? -tau : tau;
-        // reciprocal: hardware estimate refined by two Newton steps
-        double rp = __builtin_amdgcn_rcp(p);
-        rp = fma(rp, fma(-p, rp, 1.0), rp);
-        rp = fma(rp, fma(-p, rp, 1.0), rp);
-        const double l = (r > j) ? cr * rp : 0.0;
-        if (g == 0) {
-            LT[j][r] = l;
-            if (r == 0) { piv[j] = p; rpiv[j] = rp; }
-        }
-#pragma unroll
-        for (int k = 0; k < 16; ++k) a[k] = fma(-l, u[k >> 1][k & 1], a[k]);
     }
     __syncthreads();
     int pert = 0;
@@ -1028,43 +1030,101 @@ void k_diag_nopiv(float *A, int64_t lda, int nb, const float *tau_p, float *linv
     }
     if (pert) atomicAdd(npert, pert);
     __syncthreads();
-    const int c = r;
-    if (g == 0) {
-        // column c of L^-1: s = e_c; for t ascending: x_t = s_t, s_i -= L[i][t] x_t
-        double sx[NB_IN];
+    // ---- inverses, 2 x 2 blocked with 32 x 32 blocks:
+    //   L^-1 = [X11 0; X21 X22],  X21 = -X22 L21 X11
+    //   U^-1 = [Y11 Y12; 0 Y22],  Y12 = -Y11 U12 Y22
+    // the four diagonal-block inverses one per wave (lane c < 32 forms
+    // column c by right-looking substitution), then the two products by the
+    // whole workgroup
+    constexpr int H = NB_IN / 2;
+    __shared__ double Xd[4][H][H + 1];                    // X11, X22, Y11, Y22
+    __shared__ double Tp[4][H][H + 1];                    // T1, T2, X21, Y12
+    {
+        const int c = r;
+        const int o = (g & 1) * H;                        // block offset
+        if (c < H) {
+            double sv[H];
 #pragma unroll
-        for (int i = 0; i < NB_IN; ++i) sx[i] = (i == c) ? 1.0 : 0.0;
+            for (int i = 0; i < H; ++i) sv[i] = (i == c) ? 1.0 : 0.0;
+            if (g < 2) {                                  // unit lower: t ascending
 #pragma unroll
-        for (int t = 0; t < NB_IN - 1; ++t) {
+                for (int t = 0; t < H - 1; ++t) {
 #pragma unroll
-            for (int i0 = ((t + 1) / 2) * 2; i0 < NB_IN; i0 += 2) {
-                const d2_t l = *reinterpret_cast<const d2_t *>(&LT[t][i0]);
-                if (i0 > t) sx[i0] = fma(-l[0], sx[t], sx[i0]);
-                sx[i0 + 1] = fma(-l[1], sx[t], sx[i0 + 1]);
+                    for (int i0 = ((t + 1) / 2) * 2; i0 < H; i0 += 2) {
+                        const d2_t l = *reinterpret_cast<const d2_t *>(&LT[o + t][o + i0]);
+                        if (i0 > t) sv[i0] = fma(-l[0], sv[t], sv[i0]);
+                        sv[i0 + 1] = fma(-l[1], sv[t], sv[i0 + 1]);
+                    }
+                }
+            } else {                                      // upper: t descending
+#pragma unroll
+                for (int t = H - 1; t >= 0; --t) {
+                    sv[t] *= rpiv[o + t];
+#pragma unroll
+                    for (int i0 = 0; i0 < t; i0 += 2) {
+                        const d2_t u = *reinterpret_cast<const d2_t *>(&UT[o + t][o + i0]);
+                        sv[i0] = fma(-u[0], sv[t], sv[i0]);
+                        if (i0 + 1 < t) sv[i0 + 1] = fma(-u[1], sv[t], sv[i0 + 1]);
+                    }
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < H; ++i) Xd[g][i][c] = sv[i];
+        }
+    }
+    __syncthreads();
+    const int pr = tid >> 3, pc = (tid & 7) * 4;          // 32 rows x 4 columns per thread
+    // T1 = L21 X11 (L21[r][t] = L[H+r][t]),  T2 = U12 Y22 (U12[r][t] = U[r][H+t])
+    {
+        double t1[4] = {0.0, 0.0, 0.0, 0.0}, t2[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 8
+        for (int t = 0; t < H; ++t) {
+            const double lv = LT[t][H + pr], uv = UT[H + t][pr];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                t1[q] = fma(lv, Xd[0][t][pc + q], t1[q]);
+                t2[q] = fma(uv, Xd[3][t][pc + q], t2[q]);
             }
         }
 #pragma unroll
-        for (int i = 0; i < NB_IN; ++i)
-            linv[i * NB_IN + c] = (i < nb && c < nb) ? (float)sx[i] : 0.f;
-    } else if (g == 1) {
-        // column c of U^-1: s = e_c; for t descending: y_t = s_t / U[t][t],
-        // s_i -= U[i][t] y_t (i < t)
-        double sy[NB_IN];
+        for (int q = 0; q < 4; ++q) {
+            Tp[0][pr][pc + q] = t1[q];
+            Tp[1][pr][pc + q] = t2[q];
+        }
+    }
+    __syncthreads();
+    // X21 = -X22 T1,  Y12 = -Y11 T2
+    {
+        double x21[4] = {0.0, 0.0, 0.0, 0.0}, y12[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 8
+        for (int t = 0; t < H; ++t) {
+            const double xv = Xd[1][pr][t], yv = Xd[2][pr][t];
 #pragma unroll
-        for (int i = 0; i < NB_IN; ++i) sy[i] = (i == c) ? 1.0 : 0.0;
-#pragma unroll
-        for (int t = NB_IN - 1; t >= 0; --t) {
-            sy[t] *= rpiv[t];
-#pragma unroll
-            for (int i0 = 0; i0 < t; i0 += 2) {
-                const d2_t u = *reinterpret_cast<const d2_t *>(&UT[t][i0]);
-                sy[i0] = fma(-u[0], sy[t], sy[i0]);
-                if (i0 + 1 < t) sy[i0 + 1] = fma(-u[1], sy[t], sy[i0 + 1]);
+            for (int q = 0; q < 4; ++q) {
+                x21[q] = fma(xv, Tp[0][t][pc + q], x21[q]);
+                y12[q] = fma(yv, Tp[1][t][pc + q], y12[q]);
             }
         }
 #pragma unroll
-        for (int i = 0; i < NB_IN; ++i)
-            uinv[i * NB_IN + c] = (i < nb && c < nb) ? (float)sy[i] : 0.f;
+        for (int q = 0; q < 4; ++q) {
+            Tp[2][pr][pc + q] = -x21[q];
+            Tp[3][pr][pc + q] = -y12[q];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const int e = tid + 256 * k, i = e >> 6, c = e & 63;
+        const int bi = i >= H, bc = c >= H, ii = i - bi * H, cc2 = c - bc * H;
+        const bool in = i < nb && c < nb;
+        // L^-1 blocks: (0,0) X11, (1,0) X21, (1,1) X22, (0,1) zero
+        const double lval = !bi ? (!bc ? Xd[0][ii][cc2] : 0.0)
+                                : (!bc ? Tp[2][ii][cc2] : Xd[1][ii][cc2]);
+        // U^-1 blocks: (0,0) Y11, (0,1) Y12, (1,1) Y22, (1,0) zero
+        const double uval = !bi ? (!bc ? Xd[2][ii][cc2] : Tp[3][ii][cc2])
+                                : (!bc ? 0.0 : Xd[3][ii][cc2]);
+        linv[i * NB_IN + c] = in ? (float)lval : 0.f;
+        uinv[i * NB_IN + c] = in ? (float)uval : 0.f;
     }
 }
 

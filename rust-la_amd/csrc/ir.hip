@@ -477,6 +477,7 @@ struct EvTimer {
 
 // One factor-and-refine attempt (partial pivoting, or the butterfly form
 // when w.rbt); fills report and returns its status.
+constexpr int RBT_MAX_STEPS = 15;
 static int ir_attempt(la_ctx *ctx, int64_t n, int64_t nrhs, const double *A, int64_t lda,
                       const double *B, int64_t ldb, double *X, int64_t ldx, int prec,
                       double tol, int max_iter, IrWork &w, la_ir_report *report)
@@ -519,7 +520,7 @@ static int ir_attempt(la_ctx *ctx, int64_t n, int64_t nrhs, const double *A, int
     if ((st = lp_solve(ctx, n, nrhs, w, B, ldb, X, ldx))) return st;
     double h[4] = {0, 0, 0, 0};
     int it = 0;
-    double eta = INFINITY;
+    double eta = INFINITY, eta_prev = INFINITY;
     for (;; ++it) {
         if ((st = residual(ctx, n, nrhs, A, lda, X, ldx, B, ldb, w.R, nrhs, w.norms, it == 0)))
             return st;
@@ -531,6 +532,10 @@ static int ir_attempt(la_ctx *ctx, int64_t n, int64_t nrhs, const double *A, int
         eta = denom > 0 ? h[0] / denom : (h[0] == 0 ? 0.0 : INFINITY);
         if (it == 0) report->initial_backward_error = eta;
         if (!(eta > tol) || it >= max_iter || !std::isfinite(eta)) break;
+        // the unpivoted factor is abandoned for the pivoted one as soon as
+        // refinement stagnates (less than a halving per step) or runs long
+        if (w.rbt && ((it >= 2 && eta > 0.5 * eta_prev) || it >= RBT_MAX_STEPS)) break;
+        eta_prev = eta;
         if ((st = lp_solve(ctx, n, nrhs, w, w.R, nrhs, w.D, nrhs))) return st;
         hipLaunchKernelGGL(k_axpy_rows, dim3((unsigned)ceil_div(n * nrhs, 256)), dim3(256), 0,
                            ctx->stream, n, (int)nrhs, w.D, nrhs, X, ldx);

@@ -85,6 +85,26 @@ def test_rbt_singular_falls_back_and_reports(tctx, torch_cuda):
     assert rep.status in (la_hip.LA_SINGULAR, la_hip.LA_NOT_CONVERGED)
 
 
+@pytest.mark.parametrize("n,kappa", [(2000, 1e6), (1984, 1e4)])
+def test_rbt_fallback_still_meets_the_tolerance(tctx, torch_cuda, n, kappa):
+    """Systems on which the unpivoted attempt stagnates (measured): the
+    attempt is abandoned early, the pivoted factor finishes the solve, the
+    result meets the tolerance and the report names both attempts."""
+    torch = torch_cuda
+    A = torch.empty((n, n), dtype=torch.float64, device="cuda")
+    b = torch.empty((n, 1), dtype=torch.float64, device="cuda")
+    xs = torch.empty_like(b)
+    tctx.gen_svd_matrix_dev(n, kappa, 64, 2025, A, b=b, xstar=xs)
+    X = torch.empty_like(b)
+    tol = 1e-15 * n
+    rep = tctx.gesv_ir_dev(A, b, X, la_hip.PREC_FP32 | la_hip.IR_RBT, tol, 50)
+    assert rep.status == la_hip.LA_OK
+    assert rep.method == la_hip.IR_METHOD_RBT_THEN_PIVOTED
+    assert tctx.backward_error_dev(A, X, b) <= tol
+    fwd = ((X - xs).abs().max() / xs.abs().max()).item()
+    assert fwd <= 10 * kappa * 1e-15 * n
+
+
 def test_rbt_rejects_other_precisions(tctx, torch_cuda):
     torch = torch_cuda
     A, b, xs = _system(torch, tctx, 64, 1, 1e2, 4)
