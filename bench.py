@@ -167,6 +167,10 @@ def batched_lu_ir(ctx, steps, warmup, count=4096, seed=4096):
     return res
 
 
+# measured HBM bytes per launch (read + write) by kernel class, dense solve n = 16384
+MEASURED_TRAFFIC = {"gemm32": 91.87e6}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -278,7 +282,10 @@ def main():
         achieved = nbytes / calls / (avg_ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": achieved, "peak": PEAKS["hbm_gbs"], "unit": "GB/s"}
     roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["traffic"] = None
+    # HBM bytes per launch of the dominant kernel on this workload (n = 16384), from the two
+    # rocprofv3 counter passes FETCH_SIZE (x 2.000) and WRITE_SIZE (x 1.000), the factors
+    # calibrated on a 2 GiB copy in the same runs: profiles/gemm32_hbm_traffic_n16384.md
+    roof["traffic"] = MEASURED_TRAFFIC.get(name) if n == 16384 else None
     roof["kernel"] = name
     roof["avg_launch_ms"] = avg_ms
     roof["launches_per_step"] = calls
