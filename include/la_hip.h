@@ -48,9 +48,7 @@ enum la_prec {
     LA_PREC_FP64 = 0,       /* plain FP64 factor (no refinement needed)         */
     LA_PREC_FP32 = 1,       /* FP32 factor, FP32 MFMA trailing update           */
     LA_PREC_BF16 = 2,       /* BF16 operands with per-column 2^k scaling, FP32 acc */
-    LA_PREC_FP16 = 3,       /* FP16 operands with per-matrix/column 2^k scaling  */
-    LA_PREC_FP32X3 = 4      /* FP32 factor whose trailing update runs as three
-                               FP16 MFMA products (hi*hi + hi*lo + lo*hi)      */
+    LA_PREC_FP16 = 3        /* FP16 operands with per-matrix/column 2^k scaling  */
 };
 
 typedef struct la_ctx la_ctx;

@@ -28,7 +28,7 @@ LA_NOT_CONVERGED = 8
 LA_ERR_ARG = 9
 LA_ERR_TIMEOUT = 10
 
-PREC_FP64, PREC_FP32, PREC_BF16, PREC_FP16, PREC_FP32X3 = 0, 1, 2, 3, 4
+PREC_FP64, PREC_FP32, PREC_BF16, PREC_FP16 = 0, 1, 2, 3
 IR_RBT = 0x100      # option bit for gesv_ir_dev: butterfly transform, no pivoting
 IR_METHOD_PIVOTED, IR_METHOD_RBT, IR_METHOD_RBT_THEN_PIVOTED = 0, 1, 2
 
