@@ -183,8 +183,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for {args.gpus}); every GPU box here has "
-              "one GPU and the solver is single-device.", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for {args.gpus}): the solver and its "
+              "workloads are single-device.", file=sys.stderr)
         sys.exit(2)
 
     import torch
