@@ -951,11 +951,11 @@ int getrf_core(la_ctx *ctx, int64_t m, int64_t n, T *A, int64_t lda, int64_t *d_
 // FP64 factors.
 //
 // Factorisation: lane r of wave g holds the sixteen elements [r][16g ..
-// 16g+15] in registers.  Per step j (one barrier; LDS buffers
-// double-buffered by step parity; unrolled over the sixteen columns of a
-// register block so that column j's register index is static) the wave
-// holding column j publishes it, lane j of every wave publishes its row
-// segment, and every
+// 16g+15] in registers.  Per pair of steps j, j+1 (one barrier; LDS
+// buffers double-buffered by pair parity; unrolled over the sixteen columns
+// of a register block so that column j's register index is static) the
+// wave holding columns j, j+1 publishes them, lanes j and j+1 of every wave
+// publish their row segments, and every
 // element takes a[k] -= l_r * u_c with l_r = 0 for rows r <= j: one FMA
 // and no select.  Elements left of the diagonal of rows below it become
 // don't-cares; the multipliers l_r and the pivots are logged in LDS and
@@ -970,8 +970,8 @@ void k_diag_nopiv(float *A, int64_t lda, int nb, const float *tau_p, float *linv
     constexpr int LS = NB_IN + 2;                        // 16-byte aligned rows
     __shared__ __align__(16) double LT[NB_IN][LS];       // LT[t][i] = L[i][t] (i > t)
     __shared__ __align__(16) double UT[NB_IN][LS];       // UT[t][i] = U[i][t] (i <= t)
-    __shared__ __align__(16) double prow[2][NB_IN];      // row j, published by lane j
-    __shared__ double pcol[2][NB_IN];                    // column j, published by wave gj
+    __shared__ __align__(16) double prow[2][2][NB_IN];   // rows j, j+1, published by lanes j, j+1
+    __shared__ double pcol[2][2][NB_IN];                 // columns j, j+1, published by wave gj
     __shared__ double piv[NB_IN], rpiv[NB_IN];
     const int tid = threadIdx.x, r = tid & 63, g = tid >> 6;
     const double tau = fmax((double)*tau_p, 1e-30);
@@ -986,34 +986,63 @@ void k_diag_nopiv(float *A, int64_t lda, int nb, const float *tau_p, float *linv
     }
     for (int gj = 0; gj < NB_IN / 16; ++gj) {
 #pragma unroll
-        for (int kj = 0; kj < 16; ++kj) {
-            const int j = 16 * gj + kj, buf = kj & 1;
-            if (g == gj) pcol[buf][r] = a[kj];
-            if (r == j) {
+        for (int kj = 0; kj < 16; kj += 2) {
+            // steps j and j+1 share one barrier: rows and columns j, j+1 are
+            // published as they stand, and every thread forms step j's
+            // update of row and column j+1 itself (the same operations, in
+            // the same order, as two single steps)
+            const int j = 16 * gj + kj, buf = (kj >> 1) & 1;
+            if (g == gj) {
+                pcol[buf][0][r] = a[kj];
+                pcol[buf][1][r] = a[kj + 1];
+            }
+            if (r == j || r == j + 1) {
 #pragma unroll
                 for (int k = 0; k < 16; k += 2)
-                    *reinterpret_cast<d2_t *>(&prow[buf][16 * g + k]) = (d2_t){a[k], a[k + 1]};
+                    *reinterpret_cast<d2_t *>(&prow[buf][r - j][16 * g + k]) = (d2_t){a[k], a[k + 1]};
             }
             __syncthreads();
-            // every LDS read of the step is issued before the reciprocal
-            double p = pcol[buf][j];
-            const double cr = pcol[buf][r];
-            d2_t u[8];
+            // every LDS read of the pair is issued before the reciprocals
+            double p0 = pcol[buf][0][j];
+            const double q0 = pcol[buf][0][j + 1];       // a(j+1, j)
+            const double p1raw = pcol[buf][1][j + 1];    // a(j+1, j+1)
+            const double c0 = pcol[buf][0][r], c1 = pcol[buf][1][r];
+            const double s01 = prow[buf][0][j + 1];      // a(j, j+1)
+            d2_t u0[8], u1[8];
 #pragma unroll
-            for (int v = 0; v < 8; ++v)
-                u[v] = *reinterpret_cast<const d2_t *>(&prow[buf][16 * g + 2 * v]);
-            if (!(fabs(p) >= tau)) p = (p < 0.0) ? -tau : tau;
+            for (int v = 0; v < 8; ++v) {
+                u0[v] = *reinterpret_cast<const d2_t *>(&prow[buf][0][16 * g + 2 * v]);
+                u1[v] = *reinterpret_cast<const d2_t *>(&prow[buf][1][16 * g + 2 * v]);
+            }
+            if (!(fabs(p0) >= tau)) p0 = (p0 < 0.0) ? -tau : tau;
             // reciprocal: hardware estimate refined by two Newton steps
-            double rp = __builtin_amdgcn_rcp(p);
-            rp = fma(rp, fma(-p, rp, 1.0), rp);
-            rp = fma(rp, fma(-p, rp, 1.0), rp);
-            const double l = (r > j) ? cr * rp : 0.0;
+            double rp0 = __builtin_amdgcn_rcp(p0);
+            rp0 = fma(rp0, fma(-p0, rp0, 1.0), rp0);
+            rp0 = fma(rp0, fma(-p0, rp0, 1.0), rp0);
+            const double l0n = q0 * rp0;                  // row j+1's multiplier for column j
+            double p1 = fma(-l0n, s01, p1raw);
+            if (!(fabs(p1) >= tau)) p1 = (p1 < 0.0) ? -tau : tau;
+            double rp1 = __builtin_amdgcn_rcp(p1);
+            rp1 = fma(rp1, fma(-p1, rp1, 1.0), rp1);
+            rp1 = fma(rp1, fma(-p1, rp1, 1.0), rp1);
+            const double l0 = (r > j) ? c0 * rp0 : 0.0;
+            const double l1 = (r > j + 1) ? fma(-l0, s01, c1) * rp1 : 0.0;
             if (g == 0) {
-                LT[j][r] = l;
-                if (r == 0) { piv[j] = p; rpiv[j] = rp; }
+                LT[j][r] = l0;
+                LT[j + 1][r] = l1;
+                if (r == 0) {
+                    piv[j] = p0;
+                    rpiv[j] = rp0;
+                    piv[j + 1] = p1;
+                    rpiv[j + 1] = rp1;
+                }
             }
 #pragma unroll
-            for (int k = 0; k < 16; ++k) a[k] = fma(-l, u[k >> 1][k & 1], a[k]);
+            for (int k = 0; k < 16; ++k) {
+                const double u0k = u0[k >> 1][k & 1];
+                const double u1k = fma(-l0n, u0k, u1[k >> 1][k & 1]);   // row j+1 after step j
+                a[k] = fma(-l1, u1k, fma(-l0, u0k, a[k]));
+            }
         }
     }
     __syncthreads();
