@@ -1164,7 +1164,10 @@ template <typename T>
 __global__ __launch_bounds__(256)
 void k_l21_nopiv(T *A, int64_t lda, int rows, int nb, const T *__restrict__ uinv)
 {
-    __shared__ T Ui[NB_IN][NB_IN + 1];
+    // U^-1 rows 16-byte aligned: a thread's sixteen columns are four vector reads
+    constexpr int VW = 16 / (int)sizeof(T);
+    typedef T vec_t __attribute__((ext_vector_type(VW)));
+    __shared__ __align__(16) T Ui[NB_IN][NB_IN + VW];
     __shared__ T As[NB_IN][NB_IN + 1];
     const int tid = threadIdx.x;
     const int r0 = blockIdx.x * 64;
@@ -1188,8 +1191,12 @@ void k_l21_nopiv(T *A, int64_t lda, int rows, int nb, const T *__restrict__ uinv
 #pragma unroll 4
     for (int t = 0; t < NB_IN; ++t) {
         const T a = As[r][t];
+        vec_t u[16 / VW];
 #pragma unroll
-        for (int k = 0; k < 16; ++k) acc[k] = fma(a, Ui[t][g * 16 + k], acc[k]);
+        for (int v = 0; v < 16 / VW; ++v)
+            u[v] = *reinterpret_cast<const vec_t *>(&Ui[t][g * 16 + v * VW]);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) acc[k] = fma(a, u[k / VW][k % VW], acc[k]);
     }
     if (r0 + r >= rows) return;
 #pragma unroll
