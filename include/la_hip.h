@@ -40,7 +40,9 @@ enum la_status {
     LA_ERR_NOMEM = 7,       /* device allocation failed or workspace cap hit  */
     LA_NOT_CONVERGED = 8,   /* iterative refinement did not reach tolerance   */
     LA_ERR_ARG = 9,         /* null pointer, bad enum, bad leading dimension  */
-    LA_ERR_TIMEOUT = 10     /* a cross-workgroup wait exceeded its spin limit */
+    LA_ERR_TIMEOUT = 10,    /* a cross-workgroup wait exceeded its spin limit */
+    LA_ERR_IO = 11          /* reference: unwrap() panic on a missing file or
+                               an item that does not parse as f64            */
 };
 
 /* Factorisation precision for the mixed-precision solvers. */
@@ -114,16 +116,18 @@ int la_dgetrf_rm(la_ctx *ctx, int64_t m, int64_t n, const double *A,
 
 /* X (m x nx) solving L*U*X = B(piv,:) from a packed LU.
  * Replaces `LUDecomposition::solve` (src/decomp/lu.rs:237-278):
- * LA_ERR_SHAPE if b_rows != m (lu.rs:240) or m < n (the reference indexes out
- * of bounds in is_non_singular, lu.rs:174-182); LA_SINGULAR (-> None) if any
- * U diagonal entry is exactly zero (lu.rs:241-243). */
+ * LA_ERR_SHAPE if b_rows != m (lu.rs:240); then the diagonal is scanned in
+ * the order of is_non_singular (lu.rs:174-182): LA_SINGULAR (-> None) at the
+ * first exactly-zero entry lu[j][j], and, when m < n and none of the m
+ * in-bounds entries is zero, LA_ERR_SHAPE for the out-of-bounds index the
+ * reference panics on (j = m). */
 int la_dgetrs_rm(la_ctx *ctx, int64_t m, int64_t n, const double *LU,
                  const uint64_t *piv, int64_t b_rows, int64_t nx,
                  const double *B, double *X);
 
 /* getrf + getrs in one device round trip: `Matrix::solve`
- * (src/matrix/mod.rs:1030-1032).  With B = identity it is `Matrix::inverse`
- * (mod.rs:1034-1037). */
+ * (src/matrix/mod.rs:1030-1032), with la_dgetrs_rm's status order.  With
+ * B = identity it is `Matrix::inverse` (mod.rs:1034-1037). */
 int la_dgesv_rm(la_ctx *ctx, int64_t n_rows, int64_t n_cols, const double *A,
                 int64_t b_rows, int64_t nx, const double *B, double *X);
 
@@ -182,6 +186,13 @@ int la_dpotrs_dev(la_ctx *ctx, int64_t n, const double *L, int64_t ldl,
  * itself, and a refinement that does not converge is repeated with the
  * partial-pivoting factor. */
 #define LA_IR_RBT 0x100
+/* Option bit OR-ed into la_dgesv_ir_dev's prec (any precision, combinable
+ * with LA_IR_RBT): each refinement correction solves A d = r by GMRES on the
+ * system preconditioned with the low-precision factor (FP64 Krylov basis,
+ * FP64 products with A; GMRES-IR, Carson and Higham 2017/2018) instead of
+ * applying the factor once.  Converges where the factor alone is too
+ * inaccurate for plain refinement, e.g. BF16 factors at kappa = 1e6. */
+#define LA_IR_GMRES 0x200
 
 /* la_ir_report.method */
 enum la_ir_method {
@@ -204,7 +215,7 @@ typedef struct la_ir_report {
     double  total_ms;        /* convert + factor + refine (all attempts)      */
     int32_t perturbed_pivots;/* butterfly form: pivots raised to the static
                                 threshold 2^-20 max|U^T A V|                  */
-    int32_t reserved;
+    int32_t inner_iterations;/* LA_IR_GMRES: GMRES iterations, all attempts   */
 } la_ir_report;
 
 int la_dgesv_ir_dev(la_ctx *ctx, int64_t n, int64_t nrhs,
@@ -235,6 +246,8 @@ typedef struct la_batch_plan la_batch_plan;
 int la_batch_plan_create(la_ctx *ctx, int kind, int64_t count,
                          const int32_t *h_n, const int32_t *h_nrhs, int prec,
                          la_batch_plan **out);
+/* Waits for the context stream, then frees the plan.  A plan must be
+ * destroyed before the context it was created on. */
 void la_batch_plan_destroy(la_batch_plan *plan);
 /* Human-readable tile choice of the plan (autotuned or default). */
 const char *la_batch_plan_describe(la_batch_plan *plan);
@@ -311,6 +324,32 @@ int la_gen_svd_batch_dev(la_ctx *ctx, int64_t count, const int32_t *d_n,
                          const int32_t *d_nrhs, const int64_t *d_offA,
                          const int64_t *d_offB, double kappa, int nrefl, uint64_t seed,
                          uint64_t stream_base, double *A, double *B, double *Xstar);
+
+/* ------------------------------------------ input format and rendering
+ * `read_csv` (src/util.rs:9-35) for T = f64 with the item parser
+ * `str::parse::<f64>` applied to each trimmed item: one row per line
+ * (a trailing "\r" is removed with the newline), items separated by ','.
+ * LA_ERR_IO where the reference's unwrap() panics (file missing, an item
+ * that is not an f64 literal: empty items, hex floats, underscores),
+ * LA_ERR_SHAPE where its asserts panic (a row with a different item count,
+ * util.rs:25-27; an empty file, :33).  la_csv_shape counts; the other two
+ * parse into a host buffer or into device memory (row-major, ld >= cols),
+ * and return LA_ERR_ARG if rows/cols do not match the file. */
+int la_csv_shape(const char *path, int64_t *rows, int64_t *cols);
+int la_read_csv_rm(const char *path, int64_t rows, int64_t cols, double *out);
+int la_read_csv_dev(la_ctx *ctx, const char *path, int64_t rows, int64_t cols,
+                    double *d_out, int64_t ld);
+
+/* The `Debug` rendering of a Matrix<f64> (src/matrix/mod.rs:815-840, what
+ * `Matrix::print` writes): "\n", then per row '|', each element right-
+ * aligned to the widest element's width plus one leading blank, " |\n".
+ * Elements use Rust's `{:?}` for f64: the shortest round-trip digits, plain
+ * notation with at least one fractional digit for 1e-4 <= |v| < 1e16 and
+ * zero, otherwise d[.ddd]e<exp>; NaN, inf, -inf.  Writes at most cap-1
+ * characters and a terminating NUL to buf (buf may be NULL with cap 0) and
+ * returns the full length, or -1 for an empty shape. */
+int64_t la_format_debug(int64_t rows, int64_t cols, const double *data, char *buf,
+                        int64_t cap);
 
 /* ------------------------------------------------------------ utilities */
 /* eta = ||B - A X||_inf / (||A||_inf ||X||_inf + ||B||_inf), FP64 on device. */

@@ -213,6 +213,7 @@ extern "C" const char *la_status_string(int s) {
     case LA_NOT_CONVERGED: return "iterative refinement did not converge";
     case LA_ERR_ARG: return "invalid argument";
     case LA_ERR_TIMEOUT: return "device wait exceeded its spin limit";
+    case LA_ERR_IO: return "file missing or item not an f64 (reference: unwrap panic)";
     default: return "unknown status";
     }
 }
@@ -369,10 +370,19 @@ extern "C" int la_dgetrs_dev(la_ctx *ctx, int64_t m, int64_t n, const double *LU
                              const uint64_t *d_piv, int64_t nx, const double *B, int64_t ldb,
                              double *X, int64_t ldx, int32_t *d_status) {
     if (!ctx || !LU || !d_piv || !B || !X) return LA_ERR_ARG;
-    if (m < n) return set_error(ctx, LA_ERR_SHAPE, "solve: rows < cols");
     if (nx <= 0 || m <= 0 || n <= 0) return set_error(ctx, LA_ERR_SHAPE, "solve: empty");
     if (B == X) return set_error(ctx, LA_ERR_ARG, "getrs_dev: X must not alias B");
     int st;
+    if (m < n) {
+        // lu.rs:174-182: a zero among the m in-bounds diagonal entries is
+        // None; otherwise the scan indexes out of bounds (a panic)
+        int32_t *probe = d_status ? d_status : ctx->d_scratch + SCR_STATUS;
+        if ((st = check_diag_zero(ctx, m, LU, ldlu, probe))) return st;
+        int32_t h = 0;
+        if ((st = read_scratch(ctx, probe, &h, 1))) return st;
+        if (h == LA_SINGULAR) return LA_SINGULAR;
+        return set_error(ctx, LA_ERR_SHAPE, "is_non_singular: index out of bounds (rows < cols)");
+    }
     if (d_status && (st = check_diag_zero(ctx, n, LU, ldlu, d_status))) return st;
     return getrs_device(ctx, m, n, LU, ldlu, d_piv, nx, B, ldb, X, ldx);
 }
@@ -382,11 +392,15 @@ extern "C" int la_dgetrs_rm(la_ctx *ctx, int64_t m, int64_t n, const double *LU,
                             double *X) {
     if (!ctx) return LA_ERR_ARG;
     if (b_rows != m) return set_error(ctx, LA_ERR_SHAPE, "solve: b.rows() != lu.rows()");
-    if (m < n) return set_error(ctx, LA_ERR_SHAPE, "is_non_singular: index out of bounds (rows < cols)");
     if (nx <= 0 || m <= 0 || n <= 0) return set_error(ctx, LA_ERR_SHAPE, "solve: empty matrix");
     if (!LU || !piv || !B || !X) return LA_ERR_ARG;
-    for (int64_t j = 0; j < n; ++j)       // lu.rs:174-182, exact zero test
+    // lu.rs:174-182 scans j = 0..n-1 and reads lu[j*n + j]: an exact zero
+    // returns None before an out-of-bounds index (j*n + j >= m*n, first at
+    // j = m when m < n) panics.
+    for (int64_t j = 0; j < n; ++j) {
+        if (j >= m) return set_error(ctx, LA_ERR_SHAPE, "is_non_singular: index out of bounds (rows < cols)");
         if (LU[j * n + j] == 0.0) return LA_SINGULAR;
+    }
     for (int64_t i = 0; i < m; ++i)
         if (piv[i] >= (uint64_t)m) return set_error(ctx, LA_ERR_ARG, "solve: piv out of range");
     void *base = nullptr;
@@ -411,7 +425,6 @@ extern "C" int la_dgesv_rm(la_ctx *ctx, int64_t m, int64_t n, const double *A, i
     if (!ctx) return LA_ERR_ARG;
     if (m <= 0 || n <= 0 || nx <= 0) return set_error(ctx, LA_ERR_SHAPE, "solve: empty matrix");
     if (b_rows != m) return set_error(ctx, LA_ERR_SHAPE, "solve: b.rows() != a.rows()");
-    if (m < n) return set_error(ctx, LA_ERR_SHAPE, "is_non_singular: index out of bounds (rows < cols)");
     if (!A || !B || !X) return LA_ERR_ARG;
     void *base = nullptr;
     size_t total = rounded(m * n, 8) + rounded(m, 8) + 2 * rounded(m * nx, 8) + rounded(4, 4);
@@ -425,11 +438,14 @@ extern "C" int la_dgesv_rm(la_ctx *ctx, int64_t m, int64_t n, const double *A, i
     LA_HIP_CHECK(ctx, hipMemcpyAsync(dA, A, m * n * 8, hipMemcpyHostToDevice, ctx->stream));
     LA_HIP_CHECK(ctx, hipMemcpyAsync(dB, B, m * nx * 8, hipMemcpyHostToDevice, ctx->stream));
     if ((st = la_dgetrf_dev(ctx, m, n, dA, n, dpiv, dinfo))) return st;
-    if ((st = check_diag_zero(ctx, n, dA, n, dinfo + 2))) return st;
+    // the first min(m, n) diagonal entries are in bounds (lu.rs:174-182);
+    // with m < n a zero among them is None, otherwise index m panics
+    if ((st = check_diag_zero(ctx, std::min(m, n), dA, n, dinfo + 2))) return st;
     int32_t info[3];
     if ((st = read_scratch(ctx, dinfo, info, 3))) return st;
     if (info[1] != LA_OK) return set_error(ctx, info[1], "getrf: device wait failed");
     if (info[2] == LA_SINGULAR) return LA_SINGULAR;
+    if (m < n) return set_error(ctx, LA_ERR_SHAPE, "is_non_singular: index out of bounds (rows < cols)");
     if ((st = getrs_device(ctx, m, n, dA, n, dpiv, nx, dB, nx, dX, nx))) return st;
     LA_HIP_CHECK(ctx, hipMemcpyAsync(X, dX, m * nx * 8, hipMemcpyDeviceToHost, ctx->stream));
     LA_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));

@@ -186,6 +186,8 @@ extern "C" int la_batch_plan_create(la_ctx *ctx, int kind, int64_t count, const 
 extern "C" void la_batch_plan_destroy(la_batch_plan *p)
 {
     if (!p) return;
+    // a replay may still be in flight on the context stream
+    if (p->ctx && p->ctx->stream) (void)hipStreamSynchronize(p->ctx->stream);
     if (p->exec) (void)hipGraphExecDestroy(p->exec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
     if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);

@@ -186,7 +186,35 @@ void k_gemm_rm(int M, int N, int K, T alpha, const T *__restrict__ A, int64_t ld
         __syncthreads();
     }
 
-    // epilogue
+    // epilogue.  Interior tiles read C (beta != 0) sixteen elements at a
+    // time with no per-element condition, so those loads issue together;
+    // edge tiles and the lower-only (SYRK) case keep per-element guards.
+    // Both forms evaluate alpha * acc + beta * c in the same order.
+    if (full_mn && !lower_only) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            T cv[4][4];
+            T *cbase = C + (int64_t)(m0 + wm * 64 + i * 16) * ldc + n0 + wn * 64 + l16;
+            if (beta != T(0)) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        cv[r][j] = cbase[(int64_t)Mfma<T>::drow(lhi, r) * ldc + j * 16];
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                T *crow = cbase + (int64_t)Mfma<T>::drow(lhi, r) * ldc;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    T v = alpha * acc[i][j][r];
+                    if (beta != T(0)) v += beta * cv[r][j];
+                    crow[j * 16] = v;
+                }
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
 #pragma unroll

@@ -177,14 +177,27 @@ __device__ void invert_unit_lower_wave(const T *L, int ldl, int nb, T *out)
 //
 // Exchange: per column j every workgroup publishes its best candidate
 // (ordered |value| key, row index, sign) and that row's NB values, and the
-// owner of row j publishes row j.  Every published 64-bit word carries the
-// step number (the key word a step-parity bit), so a reader validates each
-// word on its own and never needs the writer to order its stores: no
-// store-completion wait and no separate flag.  Doubles travel as two words
-// (low and high halves), floats as one.  Buffers are double-buffered by
-// step parity: a workgroup reaches its step-(s+2) publication only after
-// seeing every workgroup's step-(s+1) slot, which each publishes after it
-// finished reading step s.
+// owner of row j publishes row j.  Every published 64-bit word carries a
+// tag of the global step number gs, so a reader validates each word on its
+// own and never needs the writer to order its stores: no store-completion
+// wait and no separate flag.  Value words carry gs itself in their upper
+// 32 bits; the key word has room for only ONE tag bit, bit 1 of gs.
+// Doubles travel as two words (low and high halves), floats as one.
+// Buffers are double-buffered by step parity (bit 0 of gs): a workgroup
+// reaches its step-(s+2) publication only after seeing every workgroup's
+// step-(s+1) slot, which each publishes after it finished reading step s.
+//
+// Why one key bit suffices: when a reader polls slot (buffer gs & 1,
+// workgroup q) for step gs, that slot holds either step gs or the step it
+// last validated there, gs - 2, whose bit 1 differs.  This needs (a) every
+// reader to poll the same slots at every step, (b) every panel of a
+// factorisation to write the last two steps of each slot a later panel
+// reads: panels are NB_IN = 64 columns wide except the last, and when the
+// number of workgroups P grows from one panel to the next, launch_panel
+// resets the slots first, and (c) a reset (tag 0xffffffff in every value
+// word and bit 63 set in every key word, at the start of a factorisation
+// and before a panel whose P grew) to be followed by a panel starting at a
+// step kglob = 0 mod 4, whose first two steps (bit 1 clear) reject it.
 //
 // Per column, once the winner p is known: rows j and p are swapped, then the
 // first wavefront forms the multipliers (true division, skipped for a zero
@@ -754,6 +767,7 @@ struct PanelWS {
     void *rowbuf;
     void *diagbuf;
     int64_t *swapmap;
+    int last_p = 1 << 30;       // workgroups of the previous cooperative panel
 };
 
 
@@ -761,7 +775,7 @@ constexpr size_t PANEL_LDS_MAX = 150 * 1024;
 
 template <typename T>
 static int launch_panel(la_ctx *ctx, T *Apanel, int64_t lda, int M, int NB, int kglob,
-                        int64_t *d_ipiv, const PanelWS &pw, int32_t *d_status, uint64_t ticks,
+                        int64_t *d_ipiv, PanelWS &pw, int32_t *d_status, uint64_t ticks,
                         T *linv)
 {
     // algorithmic traffic: the panel is read and written once
@@ -788,7 +802,21 @@ static int launch_panel(la_ctx *ctx, T *Apanel, int64_t lda, int M, int NB, int 
     const int rpl = std::min(rpl_max, std::max(rpl_pref, std::min(rpl_fit, rpl_max)));
     int R = 64 * rpl;
     const int P = (int)ceil_div(M, R);
-    if (P > maxp) return set_error(ctx, LA_ERR_ARG, "getrf panel: too many rows");
+    if (P > maxp)
+        return set_error(ctx, LA_ERR_ARG,
+                         "getrf: more rows than the cooperative panel holds (m > 64 * 3 * "
+                         "min(256, CUs) for FP64, 64 * 4 * min(256, CUs) for FP32)");
+    // the key word's single tag bit relies on every slot read by this panel
+    // having been written by the previous one (see the exchange comment of
+    // k_getrf_panel); when P grows, the slots are reset instead, which the
+    // first two steps reject as long as kglob is a multiple of 4
+    if (P > pw.last_p) {
+        if (kglob % 4 != 0) return set_error(ctx, LA_ERR_ARG, "getrf panel: misaligned reset");
+        int st = fill_u64(ctx, pw.slots, (int64_t)2 * PANEL_MAXP * CAND_WORDS,
+                          0xffffffff00000000ull);
+        if (st) return st;
+    }
+    pw.last_p = P;
     size_t shmem = (size_t)R * (NB + 1) * sizeof(T);
     static size_t coop_max[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
     const int ai = sizeof(T) == 8 ? 0 : 1;

@@ -19,6 +19,14 @@
 // The refinement itself (residual, update, check) is unchanged and runs on
 // the original A; a refinement that does not converge falls back to the
 // partial-pivoting factor.
+//
+// With LA_IR_GMRES each correction solves A d = r by GMRES on the left-
+// preconditioned system M^-1 A d = M^-1 r instead of d = M^-1 r, M the
+// low-precision factor (GMRES-IR, Carson and Higham, SIAM J. Sci. Comput.
+// 39(6) 2017 and 40(2) 2018).  The Krylov basis, the products with A, the
+// orthogonalisation and the update are FP64; only M^-1 applies the
+// low-precision factors.  Refinement then converges for kappa * u_f well
+// above 1 (BF16 factors at kappa = 1e6), where plain refinement diverges.
 #include "la_common.hpp"
 #include <cmath>
 #include <cstring>
@@ -96,19 +104,24 @@ __device__ inline void atomic_max_nonneg(double *addr, double v) {
 // R = B - A X for nrhs <= 16 right-hand sides.  Each wavefront takes
 // RES_RPW rows in turn, a workgroup 4 * RES_RPW rows; every lane keeps eight
 // independent loads of A (and of X) in flight.  NRB = 1 is the single
-// right-hand side without per-column guards.  norms[0] = max |R|,
-// norms[1] = max |X|, norms[2] = max |B|, norms[3] = max_i sum_j |A_ij|
-// (only when want_anorm): reduced over the workgroup first, so each
-// workgroup issues one atomic per norm.  Maxima are taken over the bit
-// patterns of the non-negative values, so a NaN (above +inf) propagates.
+// right-hand side without per-column guards.  Per column c < nrhs:
+// norms[NRM_R + c] = max_i |R_ic|, norms[NRM_X + c] = max_i |X_ic|,
+// norms[NRM_B + c] = max_i |B_ic|; norms[NRM_A] = max_i sum_j |A_ij| (only
+// when want_anorm).  Lane c of a wave carries column c, so the maxima are
+// reduced over the workgroup's waves first and each workgroup issues one
+// atomic per column and norm.  Maxima are taken over the bit patterns of
+// the non-negative values, so a NaN (above +inf) propagates.
 constexpr int RES_RPW = 4;
+// norm word layout of k_residual (NRM_WORDS doubles)
+enum { NRM_R = 0, NRM_X = 16, NRM_B = 32, NRM_A = 48, NRM_WORDS = 64 };
 template <int NRB>
 __global__ __launch_bounds__(256)
 void k_residual(int64_t n, int nrhs_arg, const double *__restrict__ A, int64_t lda,
                 const double *__restrict__ X, int64_t ldx, const double *__restrict__ B,
                 int64_t ldb, double *R, int64_t ldr, double *norms, int want_anorm)
 {
-    __shared__ unsigned long long red[4][4];
+    __shared__ unsigned long long red[4][3][16];
+    __shared__ unsigned long long reda[4];
     const int nrhs = (NRB == 1) ? 1 : nrhs_arg;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     typedef unsigned long long u64;
@@ -177,24 +190,37 @@ void k_residual(int64_t n, int nrhs_arg, const double *__restrict__ A, int64_t l
             mb = max(mb, bits(b));
         }
     }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        mr = max(mr, (u64)__shfl_xor((long long)mr, off));
-        mx = max(mx, (u64)__shfl_xor((long long)mx, off));
-        mb = max(mb, (u64)__shfl_xor((long long)mb, off));
+    if (lane < 16) {
+        red[wv][0][lane] = mr;
+        red[wv][1][lane] = mx;
+        red[wv][2][lane] = mb;
     }
-    if (lane == 0) {
-        red[wv][0] = mr;
-        red[wv][1] = mx;
-        red[wv][2] = mb;
-        red[wv][3] = ma;
-    }
+    if (lane == 0) reda[wv] = ma;
     __syncthreads();
-    if (threadIdx.x < 4) {
-        const int q = threadIdx.x;
-        const u64 v = max(max(red[0][q], red[1][q]), max(red[2][q], red[3][q]));
-        if (q < 3 || want_anorm) atomicMax(reinterpret_cast<u64 *>(norms + q), v);
+    if (threadIdx.x < 48) {
+        const int q = threadIdx.x >> 4, c = threadIdx.x & 15;
+        if (c < nrhs) {
+            const u64 v = max(max(red[0][q][c], red[1][q][c]), max(red[2][q][c], red[3][q][c]));
+            atomicMax(reinterpret_cast<u64 *>(norms + NRM_R + 16 * q + c), v);
+        }
+    } else if (threadIdx.x == 48 && want_anorm) {
+        const u64 v = max(max(reda[0], reda[1]), max(reda[2], reda[3]));
+        atomicMax(reinterpret_cast<u64 *>(norms + NRM_A), v);
     }
+}
+
+// eta = max over columns c of |R_c|max / (||A||inf |X_c|max + |B_c|max),
+// from the norm words k_residual leaves (a NaN or an Inf propagates).
+static double eta_from_norms(const double *h, int64_t nrhs, double anorm)
+{
+    double eta = 0.0;
+    for (int64_t c = 0; c < nrhs; ++c) {
+        const double r = h[NRM_R + c], denom = anorm * h[NRM_X + c] + h[NRM_B + c];
+        const double e = denom > 0 ? r / denom : (r == 0 ? 0.0 : INFINITY);
+        if (e != e) return e;            // a NaN column makes eta NaN
+        eta = std::max(eta, e);
+    }
+    return eta;
 }
 
 __global__ void k_axpy_rows(int64_t n, int nrhs, const double *D, int64_t ldd, double *X,
@@ -355,7 +381,8 @@ int residual(la_ctx *ctx, int64_t n, int64_t nrhs, const double *A, int64_t lda,
              int64_t ldr, double *d_norms, bool want_anorm)
 {
     if (nrhs > 16) return set_error(ctx, LA_ERR_ARG, "residual: nrhs > 16");
-    hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(64), 0, ctx->stream, d_norms, want_anorm ? 4 : 3);
+    hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(64), 0, ctx->stream, d_norms,
+                       want_anorm ? (int)NRM_WORDS : (int)NRM_A);
     ProfScope ps(ctx, LA_K_RESID, 2.0 * n * n * nrhs, 8.0 * ((double)n * n + 3.0 * n * nrhs));
     const unsigned grid = (unsigned)ceil_div(n, (int64_t)4 * RES_RPW);
     if (nrhs == 1)
@@ -387,7 +414,39 @@ struct IrWork {
     unsigned *amax = nullptr;
     float *tau = nullptr;
     int32_t *npert = nullptr;
+    // GMRES-IR: Krylov basis (GM_MAXV vectors of n), work vectors, dot
+    // partials and the projection coefficients
+    bool gmres = false;
+    double *V = nullptr, *gw = nullptr, *gz = nullptr, *zero = nullptr;
+    double *partial = nullptr, *hcoef = nullptr;
 };
+
+constexpr int GM_RESTART = 60;                 // Krylov vectors per correction
+constexpr int GM_MAXV = GM_RESTART + 1;
+constexpr int GM_DOT_BLOCKS = 128;             // row chunks of the dot kernel
+constexpr double GM_TOL = 1e-6;                // relative reduction per correction
+
+static int gmres_alloc(la_ctx *ctx, int64_t n, IrWork &w)
+{
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t bv = al((size_t)n * 8);
+    const size_t total = (GM_MAXV + 3) * bv + al((size_t)GM_DOT_BLOCKS * GM_MAXV * 8) +
+                         al((size_t)GM_MAXV * 8);
+    void *base = nullptr;
+    int st = ws_get(ctx, WS_IR, total, &base);
+    if (st) return st;
+    char *p = (char *)base;
+    w.V = (double *)p; p += GM_MAXV * bv;
+    w.gw = (double *)p; p += bv;
+    w.gz = (double *)p; p += bv;
+    w.zero = (double *)p; p += bv;
+    w.partial = (double *)p; p += al((size_t)GM_DOT_BLOCKS * GM_MAXV * 8);
+    w.hcoef = (double *)p;
+    hipLaunchKernelGGL(k_zero_f64, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, ctx->stream,
+                       w.zero, (int)n);
+    LA_HIP_CHECK(ctx, hipGetLastError());
+    return LA_OK;
+}
 
 static int ir_alloc(la_ctx *ctx, int64_t n, int64_t nrhs, IrWork &w)
 {
@@ -397,7 +456,7 @@ static int ir_alloc(la_ctx *ctx, int64_t n, int64_t nrhs, IrWork &w)
     size_t b32 = al((size_t)np * ld32 * sizeof(float));
     size_t bperm = al((size_t)n * 8), bipiv = al((size_t)n * 8);
     size_t brd = al((size_t)n * nrhs * 8);
-    size_t bn = al(8 * 8), bst = al(4 * 4), bsc = al((size_t)n * 8);
+    size_t bn = al(NRM_WORDS * 8), bst = al(4 * 4), bsc = al((size_t)n * 8);
     size_t bg = al((size_t)4 * np * 8), by = al((size_t)np * nrhs * 8), bsm = al(64);
     void *base = nullptr;
     int st = ws_get(ctx, WS_LP,
@@ -453,6 +512,190 @@ static int lp_solve(la_ctx *ctx, int64_t n, int64_t nrhs, const IrWork &w, const
     if (w.prec == LA_PREC_BF16 || w.prec == LA_PREC_FP16) {   // x = S y
         hipLaunchKernelGGL(k_scale_rows_by, dim3((unsigned)ceil_div(n * nrhs, 256)), dim3(256), 0,
                            ctx->stream, n, (int)nrhs, w.colscale, out, ldo);
+        LA_HIP_CHECK(ctx, hipGetLastError());
+    }
+    return LA_OK;
+}
+
+// ------------------------------------------------------------- GMRES-IR
+// partial[k * GM_DOT_BLOCKS + blk] = sum over the block's rows of V_k[i] w[i]
+// (grid: GM_DOT_BLOCKS x nv); reduced in a fixed order by k_dot_reduce, so
+// the coefficients are the same in every run.
+__global__ __launch_bounds__(256)
+void k_dot_partial(int64_t n, const double *__restrict__ V, int64_t ldv,
+                   const double *__restrict__ w, double *partial)
+{
+    __shared__ double red[4];
+    const int k = blockIdx.y;
+    const int64_t chunk = (n + GM_DOT_BLOCKS - 1) / GM_DOT_BLOCKS;
+    const int64_t i0 = (int64_t)blockIdx.x * chunk, i1 = min(n, i0 + chunk);
+    const double *v = V + (int64_t)k * ldv;
+    double acc = 0.0;
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) acc = fma(v[i], w[i], acc);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        partial[k * GM_DOT_BLOCKS + blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+__global__ void k_dot_reduce(int nv, const double *partial, double *h)
+{
+    const int k = threadIdx.x;
+    if (k >= nv) return;
+    double s = 0.0;
+    for (int b = 0; b < GM_DOT_BLOCKS; ++b) s += partial[k * GM_DOT_BLOCKS + b];
+    h[k] = s;
+}
+
+// w -= sum_k h[k] V_k  (one row per thread)
+__global__ void k_sub_basis(int64_t n, int nv, const double *__restrict__ V, int64_t ldv,
+                            const double *__restrict__ h, double *w)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double acc = w[i];
+    for (int k = 0; k < nv; ++k) acc = fma(-h[k], V[(int64_t)k * ldv + i], acc);
+    w[i] = acc;
+}
+
+// out = s * in (strided out, for a column of the correction block)
+__global__ void k_scale_vec(int64_t n, double s, const double *in, int64_t ldi, double *out,
+                            int64_t ldo)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i * ldo] = s * in[i * ldi];
+}
+
+// out[i * ldo] = sum_k y[k] V_k[i]
+__global__ void k_combine_basis(int64_t n, int nv, const double *__restrict__ V, int64_t ldv,
+                                const double *__restrict__ y, double *out, int64_t ldo)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double acc = 0.0;
+    for (int k = 0; k < nv; ++k) acc = fma(y[k], V[(int64_t)k * ldv + i], acc);
+    out[i * ldo] = acc;
+}
+
+// h[0:nv] = V[0:nv]^T x, copied to host (synchronises the stream)
+static int basis_dots(la_ctx *ctx, int64_t n, int nv, const IrWork &w, const double *x,
+                      double *h_host)
+{
+    hipLaunchKernelGGL(k_dot_partial, dim3(GM_DOT_BLOCKS, nv), dim3(256), 0, ctx->stream, n,
+                       w.V, n, x, w.partial);
+    hipLaunchKernelGGL(k_dot_reduce, dim3(1), dim3(64), 0, ctx->stream, nv, w.partial, w.hcoef);
+    LA_HIP_CHECK(ctx, hipGetLastError());
+    LA_HIP_CHECK(ctx, hipMemcpyAsync(h_host, w.hcoef, nv * sizeof(double), hipMemcpyDeviceToHost,
+                                     ctx->stream));
+    LA_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return LA_OK;
+}
+
+// ||x||_2 of a device vector (synchronises)
+static int vec_norm2(la_ctx *ctx, int64_t n, const IrWork &w, const double *x, double *nrm)
+{
+    hipLaunchKernelGGL(k_dot_partial, dim3(GM_DOT_BLOCKS, 1), dim3(256), 0, ctx->stream, n, x, n,
+                       x, w.partial);
+    hipLaunchKernelGGL(k_dot_reduce, dim3(1), dim3(64), 0, ctx->stream, 1, w.partial, w.hcoef);
+    LA_HIP_CHECK(ctx, hipGetLastError());
+    double v = 0.0;
+    LA_HIP_CHECK(ctx, hipMemcpyAsync(&v, w.hcoef, sizeof(double), hipMemcpyDeviceToHost,
+                                     ctx->stream));
+    LA_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    *nrm = std::sqrt(v);
+    return LA_OK;
+}
+
+// One GMRES-IR correction for every column c of R (n x nrhs, ld nrhs):
+// D[:, c] approximately solves A d = R[:, c]; GMRES (restarted never: at
+// most GM_RESTART iterations) on M^-1 A d = M^-1 r, classical Gram-Schmidt
+// applied twice, Givens rotations on the host.  *inner accumulates the
+// iterations taken.
+static int gmres_correction(la_ctx *ctx, int64_t n, int64_t nrhs, const double *A, int64_t lda,
+                            const IrWork &w, const double *R, double *D, int *inner)
+{
+    int st;
+    const unsigned gb = (unsigned)ceil_div(n, 256);
+    double *V = w.V;
+    for (int64_t c = 0; c < nrhs; ++c) {
+        // v_0 = z / beta, z = M^-1 r
+        if ((st = lp_solve(ctx, n, 1, w, R + c, nrhs, w.gz, 1))) return st;
+        double beta = 0.0;
+        if ((st = vec_norm2(ctx, n, w, w.gz, &beta))) return st;
+        if (!(beta > 0.0) || !std::isfinite(beta)) {
+            hipLaunchKernelGGL(k_scale_vec, dim3(gb), dim3(256), 0, ctx->stream, n, 0.0, w.gz, 1,
+                               D + c, nrhs);
+            LA_HIP_CHECK(ctx, hipGetLastError());
+            continue;
+        }
+        hipLaunchKernelGGL(k_scale_vec, dim3(gb), dim3(256), 0, ctx->stream, n, 1.0 / beta, w.gz,
+                           1, V, 1);
+        LA_HIP_CHECK(ctx, hipGetLastError());
+        // H is (GM_MAXV x GM_RESTART), column-major by iteration
+        std::vector<double> H((size_t)GM_MAXV * GM_RESTART, 0.0), cs(GM_RESTART), sn(GM_RESTART),
+            g(GM_MAXV, 0.0), h1(GM_MAXV), h2(GM_MAXV);
+        g[0] = beta;
+        int j = 0;
+        for (; j < GM_RESTART; ++j) {
+            // w = M^-1 A v_j:  R' = 0 - A v_j, then -M^-1 R'
+            if ((st = residual(ctx, n, 1, A, lda, V + (int64_t)j * n, 1, w.zero, 1, w.gw, 1,
+                               w.norms, false)))
+                return st;
+            if ((st = lp_solve(ctx, n, 1, w, w.gw, 1, w.gz, 1))) return st;
+            hipLaunchKernelGGL(k_scale_vec, dim3(gb), dim3(256), 0, ctx->stream, n, -1.0, w.gz,
+                               1, w.gz, 1);
+            LA_HIP_CHECK(ctx, hipGetLastError());
+            // classical Gram-Schmidt, twice
+            const int nv = j + 1;
+            if ((st = basis_dots(ctx, n, nv, w, w.gz, h1.data()))) return st;
+            hipLaunchKernelGGL(k_sub_basis, dim3(gb), dim3(256), 0, ctx->stream, n, nv, V, n,
+                               w.hcoef, w.gz);
+            if ((st = basis_dots(ctx, n, nv, w, w.gz, h2.data()))) return st;
+            hipLaunchKernelGGL(k_sub_basis, dim3(gb), dim3(256), 0, ctx->stream, n, nv, V, n,
+                               w.hcoef, w.gz);
+            LA_HIP_CHECK(ctx, hipGetLastError());
+            double hn = 0.0;
+            if ((st = vec_norm2(ctx, n, w, w.gz, &hn))) return st;
+            double *hc = &H[(size_t)j * GM_MAXV];
+            for (int k = 0; k < nv; ++k) hc[k] = h1[k] + h2[k];
+            hc[nv] = hn;
+            if (hn > 0.0 && std::isfinite(hn)) {
+                hipLaunchKernelGGL(k_scale_vec, dim3(gb), dim3(256), 0, ctx->stream, n, 1.0 / hn,
+                                   w.gz, 1, V + (int64_t)nv * n, 1);
+                LA_HIP_CHECK(ctx, hipGetLastError());
+            }
+            // apply the previous rotations, then form the new one
+            for (int k = 0; k < j; ++k) {
+                const double t = cs[k] * hc[k] + sn[k] * hc[k + 1];
+                hc[k + 1] = -sn[k] * hc[k] + cs[k] * hc[k + 1];
+                hc[k] = t;
+            }
+            const double rr = std::hypot(hc[j], hc[j + 1]);
+            cs[j] = rr > 0.0 ? hc[j] / rr : 1.0;
+            sn[j] = rr > 0.0 ? hc[j + 1] / rr : 0.0;
+            hc[j] = rr;
+            hc[j + 1] = 0.0;
+            g[j + 1] = -sn[j] * g[j];
+            g[j] = cs[j] * g[j];
+            ++*inner;
+            if (!(std::fabs(g[j + 1]) > GM_TOL * beta) || !(hn > 0.0) || !std::isfinite(hn)) {
+                ++j;
+                break;
+            }
+        }
+        // y = H(0:j, 0:j)^-1 g(0:j), d = V y
+        std::vector<double> y(j, 0.0);
+        for (int k = j - 1; k >= 0; --k) {
+            double t = g[k];
+            for (int q = k + 1; q < j; ++q) t -= H[(size_t)q * GM_MAXV + k] * y[q];
+            const double d = H[(size_t)k * GM_MAXV + k];
+            y[k] = d != 0.0 ? t / d : 0.0;
+        }
+        LA_HIP_CHECK(ctx, hipMemcpy(w.hcoef, y.data(), j * sizeof(double), hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_combine_basis, dim3(gb), dim3(256), 0, ctx->stream, n, j, V, n,
+                           w.hcoef, D + c, nrhs);
         LA_HIP_CHECK(ctx, hipGetLastError());
     }
     return LA_OK;
@@ -518,25 +761,29 @@ static int ir_attempt(la_ctx *ctx, int64_t n, int64_t nrhs, const double *A, int
     }
     LA_HIP_CHECK(ctx, hipEventRecord(ev.e[1], ctx->stream));
     if ((st = lp_solve(ctx, n, nrhs, w, B, ldb, X, ldx))) return st;
-    double h[4] = {0, 0, 0, 0};
+    double h[NRM_WORDS] = {};
     int it = 0;
     double eta = INFINITY, eta_prev = INFINITY;
     for (;; ++it) {
         if ((st = residual(ctx, n, nrhs, A, lda, X, ldx, B, ldb, w.R, nrhs, w.norms, it == 0)))
             return st;
-        LA_HIP_CHECK(ctx, hipMemcpyAsync(h, w.norms, 4 * sizeof(double), hipMemcpyDeviceToHost,
-                                         ctx->stream));
+        LA_HIP_CHECK(ctx, hipMemcpyAsync(h, w.norms, NRM_WORDS * sizeof(double),
+                                         hipMemcpyDeviceToHost, ctx->stream));
         LA_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-        if (it == 0) report->anorm_inf = h[3];
-        const double denom = report->anorm_inf * h[1] + h[2];
-        eta = denom > 0 ? h[0] / denom : (h[0] == 0 ? 0.0 : INFINITY);
+        if (it == 0) report->anorm_inf = h[NRM_A];
+        eta = eta_from_norms(h, nrhs, report->anorm_inf);
         if (it == 0) report->initial_backward_error = eta;
         if (!(eta > tol) || it >= max_iter || !std::isfinite(eta)) break;
         // the unpivoted factor is abandoned for the pivoted one as soon as
         // refinement stagnates (less than a halving per step) or runs long
         if (w.rbt && ((it >= 2 && eta > 0.5 * eta_prev) || it >= RBT_MAX_STEPS)) break;
         eta_prev = eta;
-        if ((st = lp_solve(ctx, n, nrhs, w, w.R, nrhs, w.D, nrhs))) return st;
+        if (w.gmres) {
+            if ((st = gmres_correction(ctx, n, nrhs, A, lda, w, w.R, w.D, &report->inner_iterations)))
+                return st;
+        } else if ((st = lp_solve(ctx, n, nrhs, w, w.R, nrhs, w.D, nrhs))) {
+            return st;
+        }
         hipLaunchKernelGGL(k_axpy_rows, dim3((unsigned)ceil_div(n * nrhs, 256)), dim3(256), 0,
                            ctx->stream, n, (int)nrhs, w.D, nrhs, X, ldx);
         LA_HIP_CHECK(ctx, hipGetLastError());
@@ -570,7 +817,8 @@ extern "C" int la_dgesv_ir_dev(la_ctx *ctx, int64_t n, int64_t nrhs, const doubl
     if (n <= 0 || nrhs <= 0 || nrhs > 16 || lda < n || ldb < nrhs || ldx < nrhs)
         return LA_ERR_ARG;
     const bool rbt = (prec & LA_IR_RBT) != 0;
-    prec &= ~LA_IR_RBT;
+    const bool gm = (prec & LA_IR_GMRES) != 0;
+    prec &= ~(LA_IR_RBT | LA_IR_GMRES);
     if (prec != LA_PREC_FP32 && prec != LA_PREC_BF16 && prec != LA_PREC_FP16)
         return set_error(ctx, LA_ERR_ARG, "dgesv_ir: precision not supported");
     if (rbt && (prec != LA_PREC_FP32 || ((n + 3) / 4) > 65535))
@@ -583,6 +831,8 @@ extern "C" int la_dgesv_ir_dev(la_ctx *ctx, int64_t n, int64_t nrhs, const doubl
     if (st) return st;
     w.prec = prec;
     w.rbt = rbt;
+    w.gmres = gm;
+    if (gm && (st = gmres_alloc(ctx, n, w))) return st;
     report->method = rbt ? LA_IR_METHOD_RBT : LA_IR_METHOD_PIVOTED;
     st = ir_attempt(ctx, n, nrhs, A, lda, B, ldb, X, ldx, prec, tol, max_iter, w, report);
     if (rbt && (st == LA_NOT_CONVERGED || st == LA_SINGULAR)) {
@@ -629,15 +879,15 @@ extern "C" int la_backward_error_dev(la_ctx *ctx, int64_t n, int64_t nrhs, const
     if (!ctx || !A || !X || !B || !eta) return LA_ERR_ARG;
     if (n <= 0 || nrhs <= 0 || nrhs > 16) return LA_ERR_ARG;
     void *base = nullptr;
-    int st = ws_get(ctx, WS_IR, (size_t)n * nrhs * 8 + 256, &base);
+    const size_t rbytes = ((size_t)n * nrhs * 8 + 255) & ~(size_t)255;
+    int st = ws_get(ctx, WS_IR, rbytes + NRM_WORDS * 8, &base);
     if (st) return st;
     double *R = (double *)base;
-    double *norms = (double *)((char *)base + (((size_t)n * nrhs * 8 + 127) & ~(size_t)127));
+    double *norms = (double *)((char *)base + rbytes);
     if ((st = residual(ctx, n, nrhs, A, lda, X, ldx, B, ldb, R, nrhs, norms, true))) return st;
-    double h[4];
-    LA_HIP_CHECK(ctx, hipMemcpyAsync(h, norms, 32, hipMemcpyDeviceToHost, ctx->stream));
+    double h[NRM_WORDS];
+    LA_HIP_CHECK(ctx, hipMemcpyAsync(h, norms, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
     LA_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    const double denom = h[3] * h[1] + h[2];
-    *eta = denom > 0 ? h[0] / denom : (h[0] == 0 ? 0.0 : INFINITY);
+    *eta = eta_from_norms(h, nrhs, h[NRM_A]);
     return LA_OK;
 }

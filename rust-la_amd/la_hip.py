@@ -8,8 +8,11 @@ tensors (torch is only the device-memory and stream plumbing here).
 There is no fallback: if the library is missing this module raises on
 import of the library, never computes on the CPU.
 """
+import atexit
 import ctypes
 import os
+import sys
+import weakref
 
 import numpy as np
 
@@ -27,9 +30,11 @@ LA_ERR_NOMEM = 7
 LA_NOT_CONVERGED = 8
 LA_ERR_ARG = 9
 LA_ERR_TIMEOUT = 10
+LA_ERR_IO = 11
 
 PREC_FP64, PREC_FP32, PREC_BF16, PREC_FP16 = 0, 1, 2, 3
 IR_RBT = 0x100      # option bit for gesv_ir_dev: butterfly transform, no pivoting
+IR_GMRES = 0x200    # option bit for gesv_ir_dev: GMRES-preconditioned corrections
 IR_METHOD_PIVOTED, IR_METHOD_RBT, IR_METHOD_RBT_THEN_PIVOTED = 0, 1, 2
 
 KERNEL_CLASSES = ["gemm64", "gemm32", "gemm16", "panel", "trsv", "resid", "trsm", "rowmove",
@@ -58,10 +63,27 @@ class IrReport(ctypes.Structure):
                 ("initial_backward_error", ctypes.c_double),
                 ("anorm_inf", ctypes.c_double), ("factor_ms", ctypes.c_double),
                 ("refine_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
-                ("perturbed_pivots", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("perturbed_pivots", ctypes.c_int32), ("inner_iterations", ctypes.c_int32)]
 
 
 _lib = None
+
+# Live contexts.  Teardown is explicit and ordered: close() releases a
+# context's plans, then the context; the atexit hook does the same for
+# whatever is still open while the HIP runtime is fully alive, and no
+# destructor calls into the library once the interpreter is finalising.
+_live_contexts = weakref.WeakSet()
+
+
+def _close_all():
+    for c in list(_live_contexts):
+        try:
+            c.close()
+        except Exception:
+            pass
+
+
+atexit.register(_close_all)
 
 
 def load():
@@ -160,17 +182,34 @@ class Context:
             pass
         self.lib = load()
         h = _vp()
+        self.h = None
+        self._plans = weakref.WeakSet()
         st = self.lib.la_ctx_create(device, ctypes.byref(h))
         if st:
             raise LaError(st, "la_ctx_create")
         self.h = h
+        _live_contexts.add(self)
 
     def close(self):
+        """Synchronise, destroy this context's batch plans, then the context."""
+        for p in list(self._plans):
+            p.close()
         if self.h:
             self.lib.la_ctx_destroy(self.h)
             self.h = None
+        _live_contexts.discard(self)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
     def __del__(self):
+        # at interpreter shutdown the atexit hook has already closed every
+        # context; never call into HIP from a finalising interpreter
+        if sys is None or sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -397,6 +436,7 @@ class BatchPlan:
 
     def __init__(self, ctx, kind, sizes, nrhs=None, prec=0):
         self.ctx = ctx
+        self.h = None
         n = np.ascontiguousarray(sizes, dtype=np.int32)
         r = np.ascontiguousarray(nrhs, dtype=np.int32) if nrhs is not None else None
         h = _vp()
@@ -405,6 +445,7 @@ class BatchPlan:
                                           prec, ctypes.byref(h))
         ctx._check(st, "batch_plan_create")
         self.h = h
+        ctx._plans.add(self)
         ctx.lib.la_batch_plan_describe.restype = ctypes.c_char_p
 
     def describe(self):
@@ -416,6 +457,8 @@ class BatchPlan:
             self.h = None
 
     def __del__(self):
+        if sys is None or sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:

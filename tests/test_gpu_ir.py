@@ -142,3 +142,45 @@ def test_bf16_ir_reports_non_convergence(tctx, torch_cuda):
     rep = tctx.gesv_ir_dev(A, b, X, la_hip.PREC_BF16, 1e-15 * n, 10)
     assert rep.status in (la_hip.LA_NOT_CONVERGED, la_hip.LA_SINGULAR)
     assert not (rep.backward_error <= 1e-15 * n)
+
+
+def _eta_per_column(a, x, b):
+    """max over columns c of ||b_c - A x_c||inf / (||A||inf ||x_c||inf + ||b_c||inf)"""
+    r = np.abs(b - a @ x).max(axis=0)
+    an = np.abs(a).sum(axis=1).max()
+    return float((r / (an * np.abs(x).max(axis=0) + np.abs(b).max(axis=0))).max())
+
+
+def test_ir_backward_error_is_per_column(tctx, torch_cuda):
+    """Right-hand sides of very different scales: the reported backward error
+    must be the maximum of the per-column backward errors (a norm taken over
+    all columns together would hide a small-scale column that is still far
+    from converged)."""
+    torch = torch_cuda
+    n, nrhs = 600, 3
+    A = torch.empty((n, n), dtype=torch.float64, device="cuda")
+    b = torch.empty((n, nrhs), dtype=torch.float64, device="cuda")
+    xs = torch.empty_like(b)
+    tctx.gen_svd_matrix_dev(n, 1e4, 8, 17, A, b=b, xstar=xs)
+    b[:, 1] *= 1e8
+    b[:, 2] *= 1e-8
+    X = torch.empty_like(b)
+    tol = 1e-15 * n
+    rep = tctx.gesv_ir_dev(A, b, X, la_hip.PREC_FP32, tol, 30)
+    torch.cuda.synchronize()
+    a, x, bb = A.cpu().numpy(), X.cpu().numpy(), b.cpu().numpy()
+    host = _eta_per_column(a, x, bb)
+    assert rep.status == la_hip.LA_OK and rep.backward_error <= tol
+    # a converged residual is rounding noise: device and host agree on the
+    # bound, not digit for digit
+    assert host <= tol, host
+    assert tctx.backward_error_dev(A, X, b) <= tol
+    # a perturbed small-scale column is seen even though the others are
+    # converged: its residual dominates its own column's backward error
+    X[:, 2] *= 1.0 + 1e-6
+    torch.cuda.synchronize()
+    host2 = _eta_per_column(a, X.cpu().numpy(), bb)
+    dev2 = tctx.backward_error_dev(A, X, b)
+    # eta of column 2 is 1e-6 relative perturbation / effective condition,
+    # far above tol; a norm over all columns together would stay near tol
+    assert host2 > 10 * tol and abs(dev2 - host2) <= 0.05 * host2, (host2, dev2)

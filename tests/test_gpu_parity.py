@@ -267,3 +267,37 @@ def test_cholesky_failures(ctx):
     d = a.copy(); d[2, 2] = np.nan           # NaN pivot carries on, as in the reference
     l, st = ctx.chol_new(d)
     assert st == la_hip.LA_OK and np.isnan(l[2, 2])
+
+
+# -------------------------------------------- status order of solve (m < n)
+def test_solve_zero_diagonal_before_out_of_bounds(ctx):
+    """lu.rs:174-182 scans lu[j*n + j] for j = 0..n-1: on a 2 x 3 factor an
+    exact zero at j = 0 returns None before the scan indexes out of bounds
+    at j = 2 (which panics)."""
+    piv = np.array([0, 1], dtype=np.uint64)
+    b = np.ones((2, 1))
+    lu = np.array([[0.0, 1.0, 2.0], [0.5, 3.0, 4.0]])
+    assert ctx.lu_solve(lu, piv, b) is None
+    lu[0, 0], lu[1, 1] = 1.0, 0.0                # zero at j = 1, still in bounds
+    assert ctx.lu_solve(lu, piv, b) is None
+    lu[1, 1] = 3.0                               # no zero: index j = 2 panics
+    with pytest.raises(la_hip.LaError) as e:
+        ctx.lu_solve(lu, piv, b)
+    assert e.value.status == la_hip.LA_ERR_SHAPE
+
+
+def test_matrix_solve_wide_status_order(ctx):
+    """Matrix::solve (mod.rs:1029-1031) on 2 x 3 inputs: a factor with a
+    zero pivot is None, otherwise the out-of-bounds scan panics; the oracle's
+    factor decides which case each input is."""
+    b = np.ones((2, 1))
+    a_sing = np.array([[0.0, 0.0, 1.0], [0.0, 0.0, 2.0]])
+    lu_o, _, _ = oracle.lu_new(a_sing)
+    assert lu_o[0, 0] == 0.0
+    assert ctx.solve(a_sing, b) is None
+    a_ok = np.array([[1.0, 2.0, 3.0], [4.0, 5.0, 6.0]])
+    lu_o, _, _ = oracle.lu_new(a_ok)
+    assert lu_o[0, 0] != 0.0 and lu_o[1, 1] != 0.0
+    with pytest.raises(la_hip.LaError) as e:
+        ctx.solve(a_ok, b)
+    assert e.value.status == la_hip.LA_ERR_SHAPE
