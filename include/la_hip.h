@@ -159,6 +159,20 @@ int la_dgemm_dev(la_ctx *ctx, int64_t m, int64_t n, int64_t k,
 int la_dgetrf_dev(la_ctx *ctx, int64_t m, int64_t n, double *A, int64_t lda,
                   uint64_t *d_piv, int32_t *d_info);
 
+/* la_dgetrf_dev without any allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * factorisation, its kernels and its results are those of la_dgetrf_dev;
+ * its workspace (panel exchange slots, swap map, L11^-1 blocks, swap list)
+ * is a fixed per-device scratch array of about 1.2 MiB instead of the
+ * context's grow-only workspace.  Limit: min(m, n) <= 65536 (LA_ERR_ARG
+ * beyond).  The scratch is shared by every call on one device and keeps the
+ * last call's contents (each call resets what it reads first): calls that
+ * overlap on one device, from any context or stream, race.  The timing
+ * harness is suspended for the call. */
+int la_dgetrf_noalloc_dev(la_ctx *ctx, int64_t m, int64_t n, double *A, int64_t lda,
+                          uint64_t *d_piv, int32_t *d_info);
+
 /* Solve with a device-resident packed LU; X (m x nx, ld ldx) must not
  * alias B.  If d_status is non-NULL it receives LA_SINGULAR when U has an
  * exact zero on its diagonal (the reference's None, lu.rs:241-243). */

@@ -158,7 +158,8 @@ extern "C" int la_ctx_create(int device, la_ctx **out) {
         return LA_ERR_NOMEM;
     }
     if (hipMemsetAsync(ctx->d_scratch, 0, 4096, ctx->stream) != hipSuccess ||
-        hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        hipStreamSynchronize(ctx->stream) != hipSuccess ||
+        noalloc_scratch_init(device) != LA_OK) {
         la_ctx_destroy(ctx);
         return LA_ERR_HIP;
     }

@@ -17,6 +17,7 @@
 //       trsm + gemm      : update the rest of the outer block
 //     trsm + gemm        : U12 = L11^-1 A12, A22 -= L21 U12 (MFMA, K = NB_OUT)
 #include "la_common.hpp"
+#include <mutex>
 
 namespace lahip {
 
@@ -877,11 +878,24 @@ static int trail_gemm(la_ctx *ctx, int upd_prec, int64_t M, int64_t N, int64_t K
     return gemm_rm<T>(ctx, M, N, K, T(-1), A, lda, B, ldb, T(1), C, ldc, false, false);
 }
 
+// Panel workspace of getrf_core: exchange slots, row buffer, diagonal-row
+// words, swap map and the ring of L11^-1 blocks, each 256-byte aligned.
+constexpr size_t ws_align(size_t x) { return (x + 255) & ~(size_t)255; }
+constexpr size_t panel_ws_bytes(size_t elem)
+{
+    return ws_align((size_t)2 * PANEL_MAXP * CAND_WORDS * sizeof(uint64_t)) + ws_align(256) +
+           ws_align((size_t)2 * 2 * NB_IN * sizeof(uint64_t)) +
+           ws_align((size_t)(1 + 4 * NB_IN) * sizeof(int64_t)) +
+           ws_align((size_t)(NB_OUT / NB_IN) * NB_IN * NB_IN * elem);
+}
+
 // In-place LU.  d_ipiv receives LAPACK-style swaps (global rows), d_perm the
 // permutation vector (may be null), d_status LA_OK / LA_ERR_TIMEOUT.
+// ext_ws: caller-provided panel workspace of panel_ws_bytes(sizeof(T))
+// bytes, or null for the context's grow-only workspace.
 template <typename T>
 int getrf_core(la_ctx *ctx, int64_t m, int64_t n, T *A, int64_t lda, int64_t *d_ipiv,
-               uint64_t *d_perm, int32_t *d_status, int upd_prec)
+               uint64_t *d_perm, int32_t *d_status, int upd_prec, void *ext_ws)
 {
     if (m > INT32_MAX / 2 || n > INT32_MAX / 2)
         return set_error(ctx, LA_ERR_ARG, "getrf: extent too large");
@@ -891,13 +905,10 @@ int getrf_core(la_ctx *ctx, int64_t m, int64_t n, T *A, int64_t lda, int64_t *d_
     size_t row_bytes = 256;
     size_t diag_bytes = (size_t)2 * 2 * NB_IN * sizeof(uint64_t);
     size_t map_bytes = (size_t)(1 + 4 * NB_IN) * sizeof(int64_t);
-    constexpr int NLINV = NB_OUT / NB_IN;
-    size_t linv_bytes = (size_t)NLINV * NB_IN * NB_IN * sizeof(T);
-    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    size_t total = al(slot_bytes) + al(row_bytes) + al(diag_bytes) + al(map_bytes) + al(linv_bytes);
-    void *base = nullptr;
-    int st = ws_get(ctx, WS_PANEL, total, &base);
-    if (st) return st;
+    auto al = [](size_t x) { return ws_align(x); };
+    void *base = ext_ws;
+    int st;
+    if (!base && (st = ws_get(ctx, WS_PANEL, panel_ws_bytes(sizeof(T)), &base))) return st;
     PanelWS pw;
     char *b = (char *)base;
     pw.slots = (uint64_t *)b; b += al(slot_bytes);
@@ -1318,13 +1329,65 @@ int pivsign(la_ctx *ctx, const int64_t *d_ipiv, int64_t kmax, int32_t *d_info)
 }
 
 template int getrf_core<double>(la_ctx *, int64_t, int64_t, double *, int64_t, int64_t *,
-                                uint64_t *, int32_t *, int);
+                                uint64_t *, int32_t *, int, void *);
 template int getrf_core<float>(la_ctx *, int64_t, int64_t, float *, int64_t, int64_t *,
-                               uint64_t *, int32_t *, int);
+                               uint64_t *, int32_t *, int, void *);
+
+// ------------------------------------------------ allocation-free entry
+// la_dgetrf_noalloc_dev takes its device workspace (the panel exchange
+// slots, swap map, L11^-1 ring and the LAPACK-style swap list) from this
+// fixed array instead of the context's grow-only workspace, so it calls no
+// allocator.  Every device that loads the library holds its own copy until
+// the process ends.  Its device address is looked up once per device (in
+// la_ctx_create, with that device current) and kept in g_scratch_ptr; host
+// code uses only that address, never the array's name.
+constexpr int64_t NOALLOC_MAX_K = 65536;
+constexpr size_t NOALLOC_PANEL_BYTES = panel_ws_bytes(sizeof(double));
+constexpr size_t NOALLOC_SCRATCH_BYTES = NOALLOC_PANEL_BYTES + NOALLOC_MAX_K * sizeof(int64_t);
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static void *g_scratch_ptr[NOALLOC_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+int noalloc_scratch_init(int device)
+{
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) return LA_ERR_ARG;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return LA_OK;
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return LA_ERR_HIP;
+    g_scratch_ptr[device] = p;
+    return LA_OK;
+}
 template int getrf_nopiv_core<float>(la_ctx *, int64_t, int64_t, float *, int64_t, const float *,
                                      int32_t *, int);
 
 }  // namespace lahip
+
+using namespace lahip;
+
+extern "C" int la_dgetrf_noalloc_dev(la_ctx *ctx, int64_t m, int64_t n, double *A, int64_t lda,
+                                     uint64_t *d_piv, int32_t *d_info)
+{
+    if (!ctx || !A || !d_piv || !d_info) return LA_ERR_ARG;
+    if (m <= 0 || n <= 0) return set_error(ctx, LA_ERR_SHAPE, "getrf: empty matrix");
+    if (lda < n) return LA_ERR_ARG;
+    const int64_t kmax = std::min(m, n);
+    if (kmax > NOALLOC_MAX_K)
+        return set_error(ctx, LA_ERR_ARG, "getrf_noalloc: min(m, n) exceeds 65536, the size "
+                                          "of the fixed scratch");
+    void *base = (ctx->device >= 0 && ctx->device < NOALLOC_MAX_DEVICES)
+                     ? g_scratch_ptr[ctx->device] : nullptr;
+    if (!base) return set_error(ctx, LA_ERR_HIP, "getrf_noalloc: scratch address not set up");
+    int64_t *ipiv = (int64_t *)((char *)base + NOALLOC_PANEL_BYTES);
+    // the timing harness creates events; keep it out of this call
+    const bool prof = ctx->prof;
+    ctx->prof = false;
+    int st = getrf_core<double>(ctx, m, n, A, lda, ipiv, d_piv, d_info + 1, LA_PREC_FP64, base);
+    if (st == LA_OK) st = pivsign(ctx, ipiv, kmax, d_info);
+    ctx->prof = prof;
+    return st;
+}
 
 #ifdef LAHIP_PANEL_TIMING
 // Probe-only export of the panel phase clock (and reset).

@@ -152,7 +152,11 @@ int gemm_rm(la_ctx *ctx, int64_t M, int64_t N, int64_t K, T alpha,
 // LA_PREC_FP16 (T = float only) = operands rounded to that format.
 template <typename T>
 int getrf_core(la_ctx *ctx, int64_t m, int64_t n, T *A, int64_t lda, int64_t *d_ipiv,
-               uint64_t *d_perm, int32_t *d_status, int upd_prec = LA_PREC_FP32);
+               uint64_t *d_perm, int32_t *d_status, int upd_prec = LA_PREC_FP32,
+               void *ext_ws = nullptr);
+// Looks up this device's copy of the fixed scratch of la_dgetrf_noalloc_dev
+// (once per device; the device must be current).
+int noalloc_scratch_init(int device);
 // LU without pivoting (static pivots below *d_tau, counted in *d_npert)
 template <typename T>
 int getrf_nopiv_core(la_ctx *ctx, int64_t m, int64_t n, T *A, int64_t lda, const float *d_tau,

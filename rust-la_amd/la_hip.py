@@ -116,6 +116,7 @@ def load():
     L.la_dgemm_dev.argtypes = [_vp, _i64, _i64, _i64, ctypes.c_double, _vp, _i64, _vp, _i64,
                                ctypes.c_double, _vp, _i64]
     L.la_dgetrf_dev.argtypes = [_vp, _i64, _i64, _vp, _i64, _vp, _vp]
+    L.la_dgetrf_noalloc_dev.argtypes = [_vp, _i64, _i64, _vp, _i64, _vp, _vp]
     L.la_dgetrs_dev.argtypes = [_vp, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _vp]
     L.la_dpotrf_dev.argtypes = [_vp, _i64, _vp, _i64, _vp]
     L.la_dpotrs_dev.argtypes = [_vp, _i64, _vp, _i64, _i64, _vp, _i64, _vp, _i64]
@@ -341,6 +342,13 @@ class Context:
         m, n = A.shape
         self._check(self.lib.la_dgetrf_dev(self.h, m, n, _ptr(A), A.stride(0), _ptr(piv),
                                            _ptr(info)), "dgetrf_dev")
+
+    def getrf_noalloc_dev(self, A, piv, info):
+        """la_dgetrf_dev's factorisation with the library's fixed scratch
+        (no allocation; min(m, n) <= 65536; overlapping calls race)."""
+        m, n = A.shape
+        self._check(self.lib.la_dgetrf_noalloc_dev(self.h, m, n, _ptr(A), A.stride(0),
+                                                   _ptr(piv), _ptr(info)), "dgetrf_noalloc_dev")
 
     def getrs_dev(self, LU, piv, B, X, status=None):
         m, n = LU.shape
