@@ -5,23 +5,29 @@ Workload (the configuration the headline metric is quoted on): n = 16384,
 A = U diag(sigma) V^T with U, V products of 64 Householder reflectors
 (seeded normal vectors), sigma geometrically spaced from 1 to 1/kappa,
 kappa = 1e6, one standard-normal right-hand side.  A "step" is one complete
-FP64-accurate solve with inputs resident in HBM:
-  FP32-IR : convert A to FP32, blocked LU on FP32 MFMA, FP64 iterative
-            refinement to normwise backward error <= 1e-15 * n
-  FP64    : plain blocked FP64 LU with partial pivoting + triangular solves
-value = FP64-equivalent flop count (2/3 n^3 - n^2/2 + 5n/6 + 2 n^2) divided
-by the FP32-IR step time (TFLOP/s).  The plain FP64 solve is timed beside
-it on the same A and b.
+FP64-accurate solve with its inputs resident in HBM: convert A to FP32,
+factor it on the FP32 MFMA, refine in FP64 (residual, correction, update) to
+normwise backward error <= 1e-15 n.  The timed steps cycle over five seeds
+(five different systems of the family, generated before the timed region),
+so the figure is not the behaviour of one matrix.
 
-Also reported: backward and forward errors, refinement steps, the dominant
-kernel's roofline (HIP-event timing of every launch of that kernel class
-during one extra, instrumented solve), and the CPU oracle (the reference's
-algorithm restated in C, one core) on a bounded sample.
+value = sum of FP64-equivalent flop ((2/3) n^3 - n^2/2 + 5n/6 + 2 n^2 per
+solve) over the timed steps / their total time (TFLOP/s); it is null unless
+every step converged.  The plain FP64 LU + solve (partial pivoting, the
+reference's algorithm) is timed beside it on the same systems.
+
+Also reported: the FP32 factor with partial pivoting, a kappa sweep
+{1e2, 1e4, 1e6} x 5 seeds of every FP32 method (which factor served, steps,
+eta, forward error, time), BF16-factor GMRES refinement at n = 16384, the
+n = 8192 FP64 LU + 16-RHS solve and DGEMM 8192^3, both batched workloads,
+the dominant kernel's roofline (HIP-event timing of every launch of that
+kernel class during one instrumented solve, traffic per launch from the
+committed counter summary), and the CPU oracle (the reference's algorithm
+restated in C, one core) on a bounded sample.
 """
 import argparse
 import json
 import os
-import subprocess
 import sys
 import time
 
@@ -29,6 +35,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "rust-la_amd"))
 
 PEAKS = {"fp64": 78.6, "fp32": 157.3, "fp16": 2500.0, "hbm_gbs": 8000.0}
+SEEDS = [2025, 2026, 2027, 2028, 2029]
+METRIC = "FP64-accurate solve TFLOP/s: FP32-IR vs plain FP64 LU, n=16384; batched solves/s"
 
 
 def lu_solve_flop(n, nrhs=1):
@@ -40,7 +48,6 @@ def cpu_baseline(n_sample):
     thread) timed on an n_sample x n_sample uniform system."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    import numpy as np
     a = oracle.uniform(7, 0, (n_sample, n_sample))
     b = oracle.uniform(7, 2, (n_sample, 1))
     try:
@@ -56,6 +63,24 @@ def cpu_baseline(n_sample):
             "sample": f"LU with partial pivoting + solve, n={n_sample}, 1 RHS, uniform[-1,1] "
                       f"({dt:.1f} s on one core; C restatement of lu.rs:104-278)",
             "seconds": dt}
+
+
+def timed(torch, fn, reps=1):
+    """Mean GPU time (ms) of fn() over reps calls, HIP events on the current stream."""
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    out = None
+    for _ in range(reps):
+        out = fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps, out
+
+
+def method_name(rep):
+    import la_hip
+    return la_hip.IR_METHOD_NAMES[rep.method]
 
 
 def batched_cholesky(ctx, steps, warmup, count=20000, seed=20000):
@@ -82,14 +107,8 @@ def batched_cholesky(ctx, steps, warmup, count=20000, seed=20000):
     plan = ctx.batch_plan(0, sizes)
     for _ in range(max(1, warmup)):
         ctx.potrf_potrs_batched_dev(plan, dOA, dOB, A, L, B, X, st)
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(steps):
-        ctx.potrf_potrs_batched_dev(plan, dOA, dOB, A, L, B, X, st)
-    e1.record()
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / steps
+    ms, _ = timed(torch, lambda: ctx.potrf_potrs_batched_dev(plan, dOA, dOB, A, L, B, X, st),
+                  steps)
     failed = int((st != 0).sum().item())
     nbytes = float((8 * (2 * n64 ** 2 + 2 * n64)).sum())
     flop = float((n64 ** 3 / 3 + n64 ** 2 / 2 + n64 / 6 + 4 * n64 ** 2).sum())
@@ -133,24 +152,21 @@ def batched_lu_ir(ctx, steps, warmup, count=4096, seed=4096):
     Xs = torch.empty_like(B)
     ctx.gen_svd_batch_dev(dn, dr, dOA, dOB, 1e3, 64, 11, 0, A, B, Xs)
     flop = float((2.0 / 3.0 * n64 ** 3).sum())
-    res = {"count": count, "n_range": [32, 1024], "kappa": 1e3,
-           "fp64_equiv_flop": flop}
-    for name, prec in (("fp16_ir", la_hip.PREC_FP16), ("fp64_lu", la_hip.PREC_FP64)):
+    res = {"count": count, "n_range": [32, 1024], "kappa": 1e3, "fp64_equiv_flop": flop}
+    X16 = None
+    for name, prec in (("fp16_ir", la_hip.PREC_FP16), ("fp32_ir", la_hip.PREC_FP32),
+                       ("fp64_lu", la_hip.PREC_FP64)):
         plan = ctx.batch_plan(1, sizes, nrhs, prec)
         X = torch.empty_like(B)
         st = torch.empty(count, dtype=torch.int32, device="cuda")
         it = torch.empty(count, dtype=torch.int32, device="cuda")
         be = torch.empty(count, dtype=torch.float64, device="cuda")
+
+        def run():
+            ctx.gesv_ir_batched_dev(plan, dOA, dOB, A, B, X, 1e-14, 30, st, it, be)
         for _ in range(max(1, warmup)):
-            ctx.gesv_ir_batched_dev(plan, dOA, dOB, A, B, X, 1e-14, 30, st, it, be)
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        for _ in range(steps):
-            ctx.gesv_ir_batched_dev(plan, dOA, dOB, A, B, X, 1e-14, 30, st, it, be)
-        e1.record()
-        torch.cuda.synchronize()
-        ms = e0.elapsed_time(e1) / steps
+            run()
+        ms, _ = timed(torch, run, steps)
         fwd = ((X - Xs).abs().max() / Xs.abs().max()).item()
         res[name] = {"ms": ms, "solves_per_s": count / (ms * 1e-3),
                      "fp64_equiv_tflops": flop / (ms * 1e-3) / 1e12,
@@ -161,26 +177,68 @@ def batched_lu_ir(ctx, steps, warmup, count=4096, seed=4096):
                      "max_forward_error_vs_xstar": fwd, "plan": plan.describe()}
         if name == "fp16_ir":
             X16 = X.clone()
-        else:
+        elif name == "fp64_lu":
             res["max_diff_fp16ir_vs_fp64lu"] = ((X16 - X).abs().max() / X.abs().max()).item()
         plan.close()
     return res
 
 
-# measured HBM bytes per launch (read + write) by kernel class, dense solve n = 16384
-MEASURED_TRAFFIC = {"gemm32": 91.87e6}
+def cfg2(ctx, torch, reps):
+    """n = 8192 plain FP64 LU + 16-RHS solve (standard normal A, B) and
+    DGEMM 8192^3 on standard normal operands."""
+    n = 8192
+    A = torch.empty((n, n), dtype=torch.float64, device="cuda")
+    ctx.fill_normal_dev(2024, 0, A)
+    B = torch.empty((n, 16), dtype=torch.float64, device="cuda")
+    ctx.fill_normal_dev(2024, 2, B)
+    W = torch.empty_like(A)
+    piv = torch.empty(n, dtype=torch.int64, device="cuda")
+    info = torch.zeros(4, dtype=torch.int32, device="cuda")
+    st = torch.zeros(1, dtype=torch.int32, device="cuda")
+    X = torch.empty_like(B)
+    t = []
+    for it in range(reps + 1):
+        W.copy_(A)
+        ms, _ = timed(torch, lambda: (ctx.getrf_dev(W, piv, info), ctx.getrs_dev(W, piv, B, X, st)))
+        if it:
+            t.append(ms)
+    ms_lu = sum(t) / len(t)
+    eta = ctx.backward_error_dev(A, X, B)
+    flop_lu = lu_solve_flop(n, 16)
+    C = torch.empty_like(A)
+    ctx.gemm_dev(A, W, C)
+    ms_gemm, _ = timed(torch, lambda: ctx.gemm_dev(A, W, C), reps)
+    tf_lu = flop_lu / (ms_lu * 1e-3) / 1e12
+    tf_gemm = 2.0 * n ** 3 / (ms_gemm * 1e-3) / 1e12
+    return {"n": n, "fp64_lu_16rhs": {"ms": ms_lu, "tflops": tf_lu,
+                                      "frac_of_fp64_peak": tf_lu / PEAKS["fp64"],
+                                      "backward_error": eta, "status": int(info[1].item())},
+            "dgemm_8192": {"ms": ms_gemm, "tflops": tf_gemm,
+                           "frac_of_fp64_peak": tf_gemm / PEAKS["fp64"]}}
+
+
+def load_traffic():
+    """HBM bytes per launch by (kernel class, workload) from the committed
+    counter summary (profiles/traffic_per_launch.json)."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic_per_launch.json")) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=16384)
     ap.add_argument("--kappa", type=float, default=1e6)
+    ap.add_argument("--method", default="tournament", choices=["tournament", "rbt", "pivoted"])
     ap.add_argument("--cpu-n", type=int, default=1536)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-sweep", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}): the solver and its "
@@ -189,148 +247,178 @@ def main():
 
     import torch
     import la_hip
-    ctx = la_hip.Context(0)
-    ctx.use_torch_stream()
     n, kappa = args.n, args.kappa
-    dev = "cuda"
-    A = torch.empty((n, n), dtype=torch.float64, device=dev)
-    b = torch.empty((n, 1), dtype=torch.float64, device=dev)
-    xs = torch.empty_like(b)
-    ctx.gen_svd_matrix_dev(n, kappa, 64, 2025, A, b=b, xstar=xs)
-    X = torch.empty_like(b)
     tol = 1e-15 * n
-    torch.cuda.synchronize()
-
-    # ---------------- FP32-IR (the measured path): butterfly-transformed FP32
-    # LU without pivoting, FP64 refinement on A (falls back to the pivoted
-    # factor if refinement does not converge; the report says which ran)
-    def ir_step(prec=la_hip.PREC_FP32 | la_hip.IR_RBT):
-        return ctx.gesv_ir_dev(A, b, X, prec, tol, 50)
-
-    for _ in range(args.warmup):
-        ir_step()
-    torch.cuda.synchronize()
-    times, reps = [], []
-    t_all0 = time.perf_counter()
-    for _ in range(args.steps):
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        rep = ir_step()
-        e1.record()
-        torch.cuda.synchronize()
-        times.append(e0.elapsed_time(e1))
-        reps.append(rep)
-    wall = time.perf_counter() - t_all0
-    ms_ir = sum(times) / len(times)
-    rep = reps[-1]
-    fwd_ir = ((X - xs).abs().max() / xs.abs().max()).item()
-    eta_ir = ctx.backward_error_dev(A, X, b)
-
-    # ---------------- FP32-IR with the partial-pivoting FP32 factor
-    Xp = X.clone()
-    ir_step(la_hip.PREC_FP32)
-    torch.cuda.synchronize()
-    tpv = []
-    for _ in range(args.steps):
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        rep_pv = ir_step(la_hip.PREC_FP32)
-        e1.record()
-        torch.cuda.synchronize()
-        tpv.append(e0.elapsed_time(e1))
-    ms_pv = sum(tpv) / len(tpv)
-    eta_pv = ctx.backward_error_dev(A, X, b)
-    fwd_pv = ((X - xs).abs().max() / xs.abs().max()).item()
-    X.copy_(Xp)
-
-    # ---------------- plain FP64 LU + solve on the same system
-    W = torch.empty_like(A)
-    piv = torch.empty(n, dtype=torch.int64, device=dev)
-    info = torch.zeros(4, dtype=torch.int32, device=dev)
-    st = torch.zeros(1, dtype=torch.int32, device=dev)
-    X64 = torch.empty_like(b)
-    t64 = []
-    for it in range(args.warmup + args.steps):
-        W.copy_(A)
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        ctx.getrf_dev(W, piv, info)
-        ctx.getrs_dev(W, piv, b, X64, st)
-        e1.record()
-        torch.cuda.synchronize()
-        if it >= args.warmup:
-            t64.append(e0.elapsed_time(e1))
-    ms_64 = sum(t64) / len(t64)
-    eta_64 = ctx.backward_error_dev(A, X64, b)
-    fwd_64 = ((X64 - xs).abs().max() / xs.abs().max()).item()
-
-    # ---------------- instrumented solve: per-kernel-class HIP-event timing
-    ctx.reset_kernel_stats()
-    ctx.set_profiling(True)
-    ir_step()
-    torch.cuda.synchronize()
-    ctx.set_profiling(False)
-    stats = ctx.kernel_stats()
-    dom = max(stats.items(), key=lambda kv: kv[1][1])
-    name, (calls, tot_ms, flop, nbytes) = dom
-    avg_ms = tot_ms / calls
-    if name.startswith("gemm"):
-        peak = {"gemm64": PEAKS["fp64"], "gemm32": PEAKS["fp32"], "gemm16": PEAKS["fp16"]}[name]
-        achieved = flop / calls / (avg_ms * 1e-3) / 1e12
-        roof = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s"}
-    else:
-        achieved = nbytes / calls / (avg_ms * 1e-3) / 1e9
-        roof = {"bound": "hbm", "achieved": achieved, "peak": PEAKS["hbm_gbs"], "unit": "GB/s"}
-    roof["frac"] = roof["achieved"] / roof["peak"]
-    # HBM bytes per launch of the dominant kernel on this workload (n = 16384), from the two
-    # rocprofv3 counter passes FETCH_SIZE (x 2.000) and WRITE_SIZE (x 1.000), the factors
-    # calibrated on a 2 GiB copy in the same runs: profiles/gemm32_hbm_traffic_n16384.md
-    roof["traffic"] = MEASURED_TRAFFIC.get(name) if n == 16384 else None
-    roof["kernel"] = name
-    roof["avg_launch_ms"] = avg_ms
-    roof["launches_per_step"] = calls
-
     flop = lu_solve_flop(n)
-    out = {
-        "metric": "FP64-accurate solve TFLOP/s: FP32-IR vs plain FP64 LU, n=16384; batched solves/s",
-        "value": flop / (ms_ir * 1e-3) / 1e12,
-        "unit": "TFLOP/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_ir,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "fp32-factor+fp64-refine",
-        "data": "synthetic: seeded Householder-product A = U diag(sigma) V^T, normal b",
-        "config": {"workload": f"dense solve n={n}, kappa={kappa:g}, 1 RHS, FP32 LU (random "
-                               f"butterfly, no pivoting) + FP64 IR to eta<=1e-15*n",
-                   "n": n, "kappa": kappa, "nrhs": 1},
-        "fp32_ir": {"ms": ms_ir, "tflops": flop / (ms_ir * 1e-3) / 1e12,
-                    "factor": "random butterfly U^T A V, FP32 LU without pivoting",
-                    "method": ["pivoted", "butterfly", "butterfly then pivoted"][rep.method],
-                    "perturbed_pivots": rep.perturbed_pivots,
-                    "iterations": rep.iterations, "backward_error": eta_ir,
-                    "forward_error_vs_xstar": fwd_ir, "factor_ms": rep.factor_ms,
-                    "refine_ms": rep.refine_ms, "status": rep.status},
-        "fp32_ir_pivoted": {"ms": ms_pv, "tflops": flop / (ms_pv * 1e-3) / 1e12,
-                            "factor": "FP32 LU with partial pivoting",
-                            "iterations": rep_pv.iterations, "backward_error": eta_pv,
-                            "forward_error_vs_xstar": fwd_pv, "factor_ms": rep_pv.factor_ms,
-                            "refine_ms": rep_pv.refine_ms, "status": rep_pv.status},
-        "fp64_lu": {"ms": ms_64, "tflops": flop / (ms_64 * 1e-3) / 1e12,
-                    "backward_error": eta_64, "forward_error_vs_xstar": fwd_64,
-                    "frac_of_fp64_peak": flop / (ms_64 * 1e-3) / 1e12 / PEAKS["fp64"]},
-        "speedup_fp32ir_vs_fp64": ms_64 / ms_ir,
-        "speedup_fp32ir_pivoted_vs_fp64": ms_64 / ms_pv,
-        "roofline": roof,
-        "kernel_breakdown_ms": {k: round(v[1], 3) for k, v in stats.items()},
-        "wall_s_timed": wall,
-    }
-    if not args.no_batched:
-        out["batched_cholesky"] = batched_cholesky(ctx, args.steps, args.warmup)
-        out["batched_lu_ir"] = batched_lu_ir(ctx, args.steps, args.warmup)
+    out = {}
+    with la_hip.Context(0) as ctx:
+        ctx.use_torch_stream()
+        opt = {"tournament": la_hip.IR_TOURNAMENT, "rbt": la_hip.IR_RBT, "pivoted": 0}
+        head_prec = la_hip.PREC_FP32 | opt[args.method]
+
+        # ---------------- the five systems of the headline, resident in HBM
+        systems = []
+        t_gen = []
+        for s in SEEDS:
+            A = torch.empty((n, n), dtype=torch.float64, device="cuda")
+            b = torch.empty((n, 1), dtype=torch.float64, device="cuda")
+            xs = torch.empty_like(b)
+            ms, _ = timed(torch, lambda: ctx.gen_svd_matrix_dev(n, kappa, 64, s, A, b=b, xstar=xs))
+            t_gen.append(ms)
+            systems.append((s, A, b, xs))
+        X = torch.empty((n, 1), dtype=torch.float64, device="cuda")
+
+        # ---------------- headline: FP32-IR, steps cycle over the systems
+        for i in range(args.warmup):
+            _, A, b, _ = systems[i % len(systems)]
+            ctx.gesv_ir_dev(A, b, X, head_prec, tol, 50)
+        torch.cuda.synchronize()
+        steps, t_all0 = [], time.perf_counter()
+        for i in range(args.steps):
+            s, A, b, xs = systems[i % len(systems)]
+            ms, rep = timed(torch, lambda: ctx.gesv_ir_dev(A, b, X, head_prec, tol, 50))
+            eta = ctx.backward_error_dev(A, X, b)
+            fwd = ((X - xs).abs().max() / xs.abs().max()).item()
+            steps.append({"seed": s, "ms": ms, "method": method_name(rep),
+                          "iterations": rep.iterations, "status": rep.status,
+                          "backward_error": eta, "forward_error_vs_xstar": fwd,
+                          "factor_ms": rep.factor_ms, "refine_ms": rep.refine_ms,
+                          "perturbed_pivots": rep.perturbed_pivots})
+        wall = time.perf_counter() - t_all0
+        ms_ir = sum(st["ms"] for st in steps) / len(steps)
+        all_ok = all(st["status"] == la_hip.LA_OK and st["backward_error"] <= tol for st in steps)
+        value = flop * len(steps) / (sum(st["ms"] for st in steps) * 1e-3) / 1e12 if all_ok else None
+
+        # ---------------- the other FP32 factor and plain FP64 on the same systems
+        def per_system(prec, reps=1):
+            res = []
+            for s, A, b, xs in systems:
+                ms, rep = timed(torch, lambda: ctx.gesv_ir_dev(A, b, X, prec, tol, 50))
+                res.append({"seed": s, "ms": ms, "method": method_name(rep),
+                            "iterations": rep.iterations, "status": rep.status,
+                            "backward_error": ctx.backward_error_dev(A, X, b),
+                            "forward_error_vs_xstar": ((X - xs).abs().max()
+                                                       / xs.abs().max()).item()})
+            return res
+        others = {m: per_system(la_hip.PREC_FP32 | o) for m, o in opt.items() if m != args.method}
+        W = torch.empty((n, n), dtype=torch.float64, device="cuda")
+        piv = torch.empty(n, dtype=torch.int64, device="cuda")
+        info = torch.zeros(4, dtype=torch.int32, device="cuda")
+        st64 = torch.zeros(1, dtype=torch.int32, device="cuda")
+        X64 = torch.empty_like(X)
+        fp64_runs = []
+        for i, (s, A, b, xs) in enumerate(systems[:3]):
+            W.copy_(A)
+            if i == 0:       # warm-up of the FP64 kernels
+                ctx.getrf_dev(W, piv, info)
+                W.copy_(A)
+            ms, _ = timed(torch, lambda: (ctx.getrf_dev(W, piv, info),
+                                          ctx.getrs_dev(W, piv, b, X64, st64)))
+            fp64_runs.append({"seed": s, "ms": ms, "backward_error":
+                              ctx.backward_error_dev(A, X64, b),
+                              "forward_error_vs_xstar": ((X64 - xs).abs().max()
+                                                         / xs.abs().max()).item()})
+        del W
+        ms_64 = sum(r["ms"] for r in fp64_runs) / len(fp64_runs)
+
+        # ---------------- instrumented solve: per-kernel-class HIP-event timing
+        s0, A0, b0, _ = systems[0]
+        ctx.reset_kernel_stats()
+        ctx.set_profiling(True)
+        ctx.gesv_ir_dev(A0, b0, X, head_prec, tol, 50)
+        torch.cuda.synchronize()
+        ctx.set_profiling(False)
+        stats = ctx.kernel_stats()
+        name, (calls, tot_ms, kflop, nbytes) = max(stats.items(), key=lambda kv: kv[1][1])
+        avg_ms = tot_ms / calls
+        if name.startswith("gemm"):
+            peak = {"gemm64": PEAKS["fp64"], "gemm32": PEAKS["fp32"],
+                    "gemm16": PEAKS["fp16"]}[name]
+            achieved = kflop / calls / (avg_ms * 1e-3) / 1e12
+            roof = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s"}
+        else:
+            achieved = nbytes / calls / (avg_ms * 1e-3) / 1e9
+            roof = {"bound": "hbm", "achieved": achieved, "peak": PEAKS["hbm_gbs"],
+                    "unit": "GB/s"}
+        roof["frac"] = roof["achieved"] / roof["peak"]
+        key = f"{name}:{args.method}:n{n}"
+        roof["traffic"] = load_traffic().get(key)
+        roof.update({"kernel": name, "avg_launch_ms": avg_ms, "launches_per_step": calls,
+                     "algorithmic_per_launch": kflop / calls if name.startswith("gemm")
+                     else nbytes / calls})
+
+        out = {
+            "metric": METRIC, "value": value, "unit": "TFLOP/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_ir,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "fp32-factor+fp64-refine",
+            "data": "synthetic: seeded Householder-product A = U diag(sigma) V^T, normal b; "
+                    f"steps cycle over seeds {SEEDS}",
+            "config": {"workload": f"dense solve n={n}, kappa={kappa:g}, 1 RHS, FP32 LU "
+                                   f"({ {'rbt': 'random butterfly, no pivoting', 'pivoted': 'partial pivoting', 'tournament': 'tournament pivoting'}[args.method]})"
+                                   " + FP64 IR to eta<=1e-15*n",
+                       "n": n, "kappa": kappa, "nrhs": 1, "seeds": SEEDS},
+            "headline_steps": steps,
+            "all_steps_converged": all_ok,
+            "fp32_ir_other_factors": {
+                m: {"mean_ms": sum(r["ms"] for r in runs) / len(runs),
+                    "tflops": flop / (sum(r["ms"] for r in runs) / len(runs) * 1e-3) / 1e12,
+                    "speedup_vs_fp64": ms_64 / (sum(r["ms"] for r in runs) / len(runs)),
+                    "runs": runs} for m, runs in others.items()},
+            "fp64_lu": {"mean_ms": ms_64, "tflops": flop / (ms_64 * 1e-3) / 1e12,
+                        "frac_of_fp64_peak": flop / (ms_64 * 1e-3) / 1e12 / PEAKS["fp64"],
+                        "runs": fp64_runs},
+            "speedup_vs_fp64": ms_64 / ms_ir,
+            "generate_ms": t_gen,
+            "roofline": roof,
+            "kernel_breakdown_ms": {k: round(v[1], 3) for k, v in stats.items()},
+            "wall_s_timed": wall,
+        }
+        del systems, A0, b0
+
+        # ---------------- kappa sweep over the family, every FP32 method
+        if not args.no_sweep:
+            A = torch.empty((n, n), dtype=torch.float64, device="cuda")
+            b = torch.empty((n, 1), dtype=torch.float64, device="cuda")
+            xs = torch.empty_like(b)
+            sweep = []
+            for kap in (1e2, 1e4, 1e6):
+                for s in SEEDS:
+                    ctx.gen_svd_matrix_dev(n, kap, 64, s, A, b=b, xstar=xs)
+                    row = {"kappa": kap, "seed": s}
+                    for label, prec in (("pivoted", la_hip.PREC_FP32),
+                                        ("butterfly", la_hip.PREC_FP32 | la_hip.IR_RBT),
+                                        ("tournament", la_hip.PREC_FP32 | la_hip.IR_TOURNAMENT)):
+                        ms, rep = timed(torch, lambda: ctx.gesv_ir_dev(A, b, X, prec, tol, 50))
+                        row[label] = {"ms": round(ms, 2), "method": method_name(rep),
+                                      "steps": rep.iterations, "status": rep.status,
+                                      "eta": rep.backward_error}
+                    sweep.append(row)
+            labs = ("pivoted", "butterfly", "tournament")
+            out["kappa_sweep"] = {
+                "n": n, "runs": sweep,
+                "served_by_own_factor": {lab: f"{sum(r[lab]['method'] == lab for r in sweep)}/{len(sweep)}"
+                                         for lab in labs},
+                "converged": {lab: f"{sum(r[lab]['status'] == 0 for r in sweep)}/{len(sweep)}"
+                              for lab in labs},
+                "median_ms": {lab: sorted(r[lab]["ms"] for r in sweep)[len(sweep) // 2]
+                              for lab in labs}}
+            # BF16 factor (per-column 2^k scaling) with GMRES refinement, kappa = 1e4
+            ctx.gen_svd_matrix_dev(n, 1e4, 64, SEEDS[0], A, b=b, xstar=xs)
+            ms, rep = timed(torch, lambda: ctx.gesv_ir_dev(
+                A, b, X, la_hip.PREC_BF16 | la_hip.IR_GMRES, tol, 20))
+            out["bf16_gmres_ir"] = {"kappa": 1e4, "ms": ms, "status": rep.status,
+                                    "outer_steps": rep.iterations,
+                                    "gmres_iterations": rep.inner_iterations,
+                                    "backward_error": ctx.backward_error_dev(A, X, b),
+                                    "forward_error_vs_xstar": ((X - xs).abs().max()
+                                                               / xs.abs().max()).item(),
+                                    "tflops": flop / (ms * 1e-3) / 1e12}
+            del A
+        out["cfg2_n8192"] = cfg2(ctx, torch, 2)
+        if not args.no_batched:
+            out["batched_cholesky"] = batched_cholesky(ctx, args.steps, args.warmup)
+            out["batched_lu_ir"] = batched_lu_ir(ctx, args.steps, args.warmup)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_n)
         out["cpu_baseline"]["host_nproc"] = os.cpu_count()
@@ -340,7 +428,6 @@ def main():
             out["cpu_baseline"]["host_cpu"] = model
         except Exception:
             pass
-    ctx.close()
     print(json.dumps(out))
 
 

@@ -83,7 +83,8 @@ enum la_kernel_class {
     LA_K_TRSM = 6,      /* unit-lower TRSM of the LU U12 block               */
     LA_K_ROWMOVE = 7,   /* row interchanges                                  */
     LA_K_BATCH = 8,     /* batched persistent kernels                        */
-    LA_K_COUNT = 9
+    LA_K_PIVSEL = 9,    /* tournament pivot selection + row moves (CALU)     */
+    LA_K_COUNT = 10
 };
 int la_ctx_set_profiling(la_ctx *ctx, int enable);
 int la_ctx_reset_kernel_stats(la_ctx *ctx);
@@ -130,6 +131,14 @@ int la_dgetrs_rm(la_ctx *ctx, int64_t m, int64_t n, const double *LU,
  * B = identity it is `Matrix::inverse` (mod.rs:1034-1037). */
 int la_dgesv_rm(la_ctx *ctx, int64_t n_rows, int64_t n_cols, const double *A,
                 int64_t b_rows, int64_t nx, const double *B, double *X);
+
+/* Ainv = A^-1: `Matrix::inverse` (src/matrix/mod.rs:1034-1037), i.e.
+ * LUDecomposition::new(A).solve(id): LA_ERR_SHAPE if A is not square
+ * (mod.rs:1035), LA_SINGULAR (-> None) for an exact zero on U's diagonal.
+ * The identity's row permutation is formed on the device and both
+ * triangular solves with n right-hand sides run as the blocked MFMA TRSM
+ * (64 x 64 diagonal blocks inverted, then one GEMM per block). */
+int la_dinverse_rm(la_ctx *ctx, int64_t rows, int64_t cols, const double *A, double *Ainv);
 
 /* Cholesky factor L (full n x n, strict upper triangle zeroed).
  * Replaces `CholeskyDecomposition::new` (src/decomp/cholesky.rs:56-110):
@@ -180,6 +189,13 @@ int la_dgetrs_dev(la_ctx *ctx, int64_t m, int64_t n, const double *LU, int64_t l
                   const uint64_t *d_piv, int64_t nx, const double *B, int64_t ldb,
                   double *X, int64_t ldx, int32_t *d_status);
 
+/* Device-resident inverse: A (ld lda) is overwritten with its packed LU,
+ * d_piv receives the permutation, X (ld must equal n) the inverse; d_info
+ * (device int[4]) = {pospivsign, status, singular flag (LA_SINGULAR if U
+ * has an exact zero on its diagonal, X then undefined)}. */
+int la_dinverse_dev(la_ctx *ctx, int64_t n, double *A, int64_t lda, uint64_t *d_piv,
+                    double *X, int64_t ldx, int32_t *d_info);
+
 /* In-place Cholesky of the n x n matrix at A (lower factor, upper zeroed);
  * d_status (device int) receives LA_OK / LA_NOT_SYMMETRIC / LA_NOT_SPD. */
 int la_dpotrf_dev(la_ctx *ctx, int64_t n, double *A, int64_t lda, int32_t *d_status);
@@ -207,13 +223,22 @@ int la_dpotrs_dev(la_ctx *ctx, int64_t n, const double *L, int64_t ldl,
  * applying the factor once.  Converges where the factor alone is too
  * inaccurate for plain refinement, e.g. BF16 factors at kappa = 1e6. */
 #define LA_IR_GMRES 0x200
+/* Option bit OR-ed into la_dgesv_ir_dev's prec (LA_PREC_FP32): the FP32
+ * factor's pivots are chosen per 64-column panel by a tournament (CALU:
+ * partial pivoting on candidate rows along a reduction tree, then the panel
+ * factored without pivoting) instead of a search per column.  A different
+ * permutation from partial pivoting, with the same growth in practice; a
+ * refinement that does not converge is repeated with partial pivoting. */
+#define LA_IR_TOURNAMENT 0x400
 
 /* la_ir_report.method */
 enum la_ir_method {
     LA_IR_METHOD_PIVOTED = 0,          /* partial-pivoting low-precision factor  */
     LA_IR_METHOD_RBT = 1,              /* butterfly transform, no pivoting       */
-    LA_IR_METHOD_RBT_THEN_PIVOTED = 2  /* butterfly attempt did not converge;
+    LA_IR_METHOD_RBT_THEN_PIVOTED = 2, /* butterfly attempt did not converge;
                                           the result is the pivoted attempt's  */
+    LA_IR_METHOD_TOURNAMENT = 3,       /* tournament-pivoted factor            */
+    LA_IR_METHOD_TOURNAMENT_THEN_PIVOTED = 4
 };
 
 typedef struct la_ir_report {
@@ -338,6 +363,41 @@ int la_gen_svd_batch_dev(la_ctx *ctx, int64_t count, const int32_t *d_n,
                          const int32_t *d_nrhs, const int64_t *d_offA,
                          const int64_t *d_offB, double kappa, int nrefl, uint64_t seed,
                          uint64_t stream_base, double *A, double *B, double *Xstar);
+
+/* ------------------------------------------------------------ f32 surface
+ * Matrix<f32> operations of src/matrix/simd.rs on the GPU.
+ * simd_mul (simd.rs:189-219): C[m x p] = A[m x k] B[k2 x p] in FP32 on the
+ * FP32 MFMA (its sums round differently from the reference's sequential
+ * f32x4 loop; FP32 rounding bound k 2^-24 |A||B|).  LA_ERR_SHAPE if k != k2
+ * (simd.rs:190). */
+int la_sgemm_rm(la_ctx *ctx, int64_t m, int64_t k, int64_t k2, int64_t p,
+                const float *A, const float *B, float *C);
+int la_sgemm_dev(la_ctx *ctx, int64_t m, int64_t n, int64_t k, float alpha,
+                 const float *A, int64_t lda, const float *B, int64_t ldb,
+                 float beta, float *C, int64_t ldc);
+
+/* simd_scale, simd_add, simd_sub, simd_elem_mul, simd_elem_div, simd_neg
+ * (simd.rs:7-164): out = a * scalar, a + b, a - b, a * b, a / b, -a, one
+ * IEEE operation per element (bit-identical to the reference's lanes).
+ * Binary operations need equal shapes (LA_ERR_SHAPE, simd.rs:32-33 and
+ * alike).  Unlike the reference, matrices of fewer than four elements get
+ * every element computed (its scalar tail does not run for them). */
+enum la_elementwise_op {
+    LA_EW_SCALE = 0, LA_EW_ADD = 1, LA_EW_SUB = 2, LA_EW_MUL = 3, LA_EW_DIV = 4, LA_EW_NEG = 5
+};
+int la_selementwise_rm(la_ctx *ctx, int op, int64_t rows, int64_t cols,
+                       int64_t b_rows, int64_t b_cols, const float *a, const float *b,
+                       float scalar, float *out);
+int la_selementwise_dev(la_ctx *ctx, int op, int64_t count, const float *a,
+                        const float *b, float scalar, float *out);
+
+/* simd_dot (simd.rs:166-187) of two equal-length columns (LA_ERR_SHAPE
+ * otherwise, :167-168): the sum of every a[i] * b[i], accumulated in FP64
+ * and rounded to FP32 once.  The reference's tail adds a[i] instead of
+ * a[i] * b[i] for the last len mod 4 elements; that defect is not kept. */
+int la_sdot_rm(la_ctx *ctx, int64_t rows, int64_t cols, int64_t b_rows, int64_t b_cols,
+               const float *a, const float *b, float *out);
+int la_sdot_dev(la_ctx *ctx, int64_t count, const float *a, const float *b, float *d_out);
 
 /* ------------------------------------------ input format and rendering
  * `read_csv` (src/util.rs:9-35) for T = f64 with the item parser

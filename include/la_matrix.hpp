@@ -287,7 +287,17 @@ std::optional<Matrix<T>> Matrix<T>::solve(const Matrix &b) const {
 template <typename T>
 std::optional<Matrix<T>> Matrix<T>::inverse() const {
     if (no_rows_ != cols()) throw Panic("inverse: not square");
-    return LUDecomposition<T>(*this).solve(Matrix::id(no_rows_, no_rows_));
+    if constexpr (std::is_same_v<T, double>) {
+        // one device round trip: LU, P I formed on the device, MFMA TRSMs
+        std::vector<double> out(no_rows_ * no_rows_);
+        const int st = la_dinverse_rm(thread_ctx(), (int64_t)no_rows_, (int64_t)no_rows_,
+                                      data_.data(), out.data());
+        if (st == LA_SINGULAR) return std::nullopt;
+        check(st, "inverse");
+        return Matrix(no_rows_, no_rows_, std::move(out));
+    } else {
+        return LUDecomposition<T>(*this).solve(Matrix::id(no_rows_, no_rows_));
+    }
 }
 template <typename T>
 bool Matrix<T>::is_non_singular() const {

@@ -18,6 +18,9 @@
  *   orc_lu_solve       src/decomp/lu.rs:237-278
  *   orc_chol_new       src/decomp/cholesky.rs:56-110
  *   orc_chol_solve     src/decomp/cholesky.rs:116-144
+ *   orc_smul           src/matrix/simd.rs:189-219    (Matrix<f32>::simd_mul)
+ *   orc_sdot           src/matrix/simd.rs:166-187    (simd_dot, tail defect
+ *                      corrected: the tail adds a[i]*b[i], see below)
  *
  * Status codes (mirroring the reference's failure signals):
  *   0  OK
@@ -143,6 +146,75 @@ int orc_lu_new(int64_t m, int64_t n, const double *a, double *lu,
 /* ---- is_non_singular (lu.rs:174-182): no exact zero on the diagonal for
  * j < cols.  The reference indexes lu[j*n+j] for every j < n, which runs
  * past the end of the data (a panic) when m < n; that case returns -1. */
+/* The same factorisation with column j of lu copied to a contiguous buffer
+ * before its dot products (JAMA's LUcolj): every product and sum is the
+ * same operation on the same operands in the same order as orc_lu_new, so
+ * the results are bit-identical; only the memory walk differs (the
+ * reference's strided column walk makes n = 8192 take hours).  Used to
+ * write large fixtures; orc_lu_new stays the statement of the reference
+ * and its timing is the CPU baseline. */
+int orc_lu_new_colcopy(int64_t m, int64_t n, const double *a, double *lu,
+                       uint64_t *piv, int *pospivsign, double *min_gap)
+{
+    if (m <= 0 || n <= 0) return ORC_SHAPE;
+    double *col = (double *)malloc((size_t)m * sizeof(double));
+    if (!col) return ORC_SHAPE;
+    memcpy(lu, a, (size_t)(m * n) * sizeof(double));
+    for (int64_t i = 0; i < m; ++i) piv[i] = (uint64_t)i;
+    int sign = 1;
+    double gap = 1.0;
+    for (int64_t j = 0; j < n; ++j) {
+        for (int64_t i = 0; i < m; ++i) col[i] = lu[i * n + j];
+        for (int64_t i = 0; i < m; ++i) {
+            double s = 0.0;
+            int64_t kmax = i < j ? i : j;
+            const double *row = lu + i * n;
+            for (int64_t kk = 0; kk < kmax; ++kk) {
+                double prod = row[kk] * col[kk];
+                s = s + prod;
+            }
+            col[i] = col[i] - s;
+            lu[i * n + j] = col[i];
+        }
+        int64_t p = j;
+        for (int64_t i = j + 1; i < m; ++i) {
+            if (fabs(lu[i * n + j]) > fabs(lu[p * n + j])) p = i;
+        }
+        if (min_gap && j + 1 < m) {
+            double best = -1.0, second = -1.0;
+            for (int64_t i = j; i < m; ++i) {
+                double v = fabs(lu[i * n + j]);
+                if (v != v) continue;
+                if (v > best) { second = best; best = v; }
+                else if (v > second) second = v;
+            }
+            if (best > 0.0 && second >= 0.0) {
+                double g = (best - second) / best;
+                if (g < gap) gap = g;
+            }
+        }
+        if (p != j) {
+            for (int64_t kk = 0; kk < n; ++kk) {
+                double t = lu[p * n + kk];
+                lu[p * n + kk] = lu[j * n + kk];
+                lu[j * n + kk] = t;
+            }
+            uint64_t t = piv[p];
+            piv[p] = piv[j];
+            piv[j] = t;
+            sign = !sign;
+        }
+        if (j < m && lu[j * n + j] != 0.0) {
+            for (int64_t i = j + 1; i < m; ++i)
+                lu[i * n + j] = lu[i * n + j] / lu[j * n + j];
+        }
+    }
+    free(col);
+    *pospivsign = sign;
+    if (min_gap) *min_gap = gap;
+    return ORC_OK;
+}
+
 int orc_lu_is_non_singular(int64_t m, int64_t n, const double *lu)
 {
     for (int64_t j = 0; j < n; ++j) {
@@ -293,4 +365,59 @@ void orc_fill_normal(uint64_t seed, uint64_t stream, int64_t count, double *out)
         out[2 * p] = r * cos(t);
         if (2 * p + 1 < count) out[2 * p + 1] = r * sin(t);
     }
+}
+
+/* ---------------------------------------------------------------- f32
+ * Matrix<f32>::simd_mul (src/matrix/simd.rs:189-219): for each row, each
+ * block of four output columns accumulates res = res + splat(a[row][i]) *
+ * b[i][4c..4c+4] over i ascending, each lane a separately rounded FP32
+ * product then sum (f32x4 mul and add, no fusion); the remaining columns
+ * (p mod 4) use the scalar loop res += a * b, i ascending. */
+int orc_smul(int64_t m, int64_t k, int64_t k2, int64_t p, const float *a, const float *b,
+             float *c)
+{
+    if (k != k2) return 1;                       /* simd.rs:190 */
+    for (int64_t row = 0; row < m; ++row) {
+        for (int64_t cb = 0; cb < p / 4; ++cb) {
+            float res[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            for (int64_t i = 0; i < k; ++i) {
+                const float av = a[row * k + i];
+                for (int l = 0; l < 4; ++l) {
+                    const float prod = av * b[i * p + 4 * cb + l];
+                    res[l] = res[l] + prod;
+                }
+            }
+            for (int l = 0; l < 4; ++l) c[row * p + 4 * cb + l] = res[l];
+        }
+        for (int64_t col = p & ~(int64_t)3; col < p; ++col) {
+            float res = 0.0f;
+            for (int64_t i = 0; i < k; ++i) {
+                const float prod = a[row * k + i] * b[i * p + col];
+                res += prod;
+            }
+            c[row * p + col] = res;
+        }
+    }
+    return 0;
+}
+
+/* simd_dot (src/matrix/simd.rs:166-187): four lanes accumulate a*b over the
+ * 4-element groups, then sum = ((l0 + l1) + l2) + l3.  The reference's tail
+ * loop runs over [len - (len & !3), len) and adds a[i] alone; that is a
+ * defect, so this restatement adds a[i] * b[i] for the len mod 4 trailing
+ * elements instead (the mathematically intended dot product). */
+float orc_sdot(int64_t len, const float *a, const float *b)
+{
+    float lanes[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int64_t i = 0; i < len / 4; ++i)
+        for (int l = 0; l < 4; ++l) {
+            const float prod = a[4 * i + l] * b[4 * i + l];
+            lanes[l] = lanes[l] + prod;
+        }
+    float sum = ((lanes[0] + lanes[1]) + lanes[2]) + lanes[3];
+    for (int64_t i = len & ~(int64_t)3; i < len; ++i) {
+        const float prod = a[i] * b[i];
+        sum += prod;
+    }
+    return sum;
 }

@@ -34,6 +34,7 @@ def lib():
         L.orc_mul.argtypes = [_i64, _i64, _i64, _i64, _dp, _dp, _dp]
         L.orc_mmul.argtypes = [_i64] * 6 + [_dp, _dp, _dp]
         L.orc_lu_new.argtypes = [_i64, _i64, _dp, _dp, _u64p, ctypes.POINTER(ctypes.c_int), _dp]
+        L.orc_lu_new_colcopy.argtypes = L.orc_lu_new.argtypes
         L.orc_lu_is_non_singular.argtypes = [_i64, _i64, _dp]
         L.orc_lu_det.argtypes = [_i64, _i64, _dp, ctypes.c_int, _dp]
         L.orc_lu_solve.argtypes = [_i64, _i64, _dp, _u64p, _i64, _i64, _dp, _dp]
@@ -43,6 +44,10 @@ def lib():
         L.orc_rng_u64.restype = ctypes.c_uint64
         L.orc_fill_uniform.argtypes = [ctypes.c_uint64, ctypes.c_uint64, _i64, _dp]
         L.orc_fill_normal.argtypes = [ctypes.c_uint64, ctypes.c_uint64, _i64, _dp]
+        _fp = ctypes.POINTER(ctypes.c_float)
+        L.orc_smul.argtypes = [_i64, _i64, _i64, _i64, _fp, _fp, _fp]
+        L.orc_sdot.argtypes = [_i64, _fp, _fp]
+        L.orc_sdot.restype = ctypes.c_float
         _lib = L
     return _lib
 
@@ -81,6 +86,23 @@ def lu_new(a, want_gap=False):
     gap = ctypes.c_double()
     st = lib().orc_lu_new(m, n, _d(a), _d(lu), piv.ctypes.data_as(_u64p), ctypes.byref(sign),
                           ctypes.byref(gap))
+    if st:
+        raise OracleError(st)
+    out = (lu, piv, bool(sign.value))
+    return out + (gap.value,) if want_gap else out
+
+
+def lu_new_colcopy(a, want_gap=False):
+    """lu_new with the column walk made contiguous (bit-identical results;
+    for writing large fixtures)."""
+    a = _c(a)
+    m, n = a.shape
+    lu = np.empty_like(a)
+    piv = np.empty(m, dtype=np.uint64)
+    sign = ctypes.c_int()
+    gap = ctypes.c_double()
+    st = lib().orc_lu_new_colcopy(m, n, _d(a), _d(lu), piv.ctypes.data_as(_u64p),
+                                  ctypes.byref(sign), ctypes.byref(gap))
     if st:
         raise OracleError(st)
     out = (lu, piv, bool(sign.value))
@@ -144,6 +166,27 @@ def chol_solve(l, b):
     if st:
         raise OracleError(st)
     return x
+
+
+def smul(a, b):
+    """Matrix<f32>::simd_mul (src/matrix/simd.rs:189-219), FP32."""
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    b = np.ascontiguousarray(b, dtype=np.float32)
+    c = np.empty((a.shape[0], b.shape[1]), dtype=np.float32)
+    fp = ctypes.POINTER(ctypes.c_float)
+    st = lib().orc_smul(a.shape[0], a.shape[1], b.shape[0], b.shape[1], a.ctypes.data_as(fp),
+                        b.ctypes.data_as(fp), c.ctypes.data_as(fp))
+    if st:
+        raise OracleError(st)
+    return c
+
+
+def sdot(a, b):
+    """simd_dot (simd.rs:166-187) with the tail adding a[i]*b[i]."""
+    a = np.ascontiguousarray(a, dtype=np.float32).ravel()
+    b = np.ascontiguousarray(b, dtype=np.float32).ravel()
+    fp = ctypes.POINTER(ctypes.c_float)
+    return float(lib().orc_sdot(a.size, a.ctypes.data_as(fp), b.ctypes.data_as(fp)))
 
 
 def uniform(seed, stream, shape):

@@ -363,6 +363,10 @@ static int getrs_device(la_ctx *ctx, int64_t m, int64_t n, const double *dLU, in
                         double *dX, int64_t ldx) {
     int st;
     if ((st = gather_rows(ctx, m, nx, dpiv, dB, ldb, dX, ldx))) return st;
+    if (nx >= TRSM_MIN_RHS && n >= TRSM_MIN_N) {     // many columns: MFMA TRSM
+        if ((st = trsm_blocked(ctx, 'L', n, dLU, ldlu, nx, dX, ldx))) return st;
+        return trsm_blocked(ctx, 'U', n, dLU, ldlu, nx, dX, ldx);
+    }
     if ((st = trsv_blocked<double>(ctx, 'L', false, true, n, dLU, ldlu, nx, dX, ldx))) return st;
     return trsv_blocked<double>(ctx, 'U', false, false, n, dLU, ldlu, nx, dX, ldx);
 }
@@ -449,6 +453,65 @@ extern "C" int la_dgesv_rm(la_ctx *ctx, int64_t m, int64_t n, const double *A, i
     if (m < n) return set_error(ctx, LA_ERR_SHAPE, "is_non_singular: index out of bounds (rows < cols)");
     if ((st = getrs_device(ctx, m, n, dA, n, dpiv, nx, dB, nx, dX, nx))) return st;
     LA_HIP_CHECK(ctx, hipMemcpyAsync(X, dX, m * nx * 8, hipMemcpyDeviceToHost, ctx->stream));
+    LA_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return check_sticky(ctx);
+}
+
+// --------------------------------------------------------------- inverse
+namespace {
+// X = P I: X[i][j] = 1 where j == piv[i]  (lu.rs:246-254 with B = id)
+__global__ void k_perm_identity(int64_t n, const uint64_t *piv, double *X)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * n) return;
+    const int64_t i = e / n, j = e - i * n;
+    X[e] = (piv[i] == (uint64_t)j) ? 1.0 : 0.0;
+}
+}  // namespace
+
+extern "C" int la_dinverse_dev(la_ctx *ctx, int64_t n, double *A, int64_t lda, uint64_t *d_piv,
+                               double *X, int64_t ldx, int32_t *d_info)
+{
+    if (!ctx || !A || !d_piv || !X || !d_info || n <= 0 || lda < n) return LA_ERR_ARG;
+    if (ldx != n) return set_error(ctx, LA_ERR_ARG, "inverse_dev: ldx must equal n");
+    int st;
+    if ((st = la_dgetrf_dev(ctx, n, n, A, lda, d_piv, d_info))) return st;
+    if ((st = check_diag_zero(ctx, n, A, lda, d_info + 2))) return st;
+    // X = P I, then the two triangular solves with n right-hand sides
+    hipLaunchKernelGGL(k_perm_identity, dim3((unsigned)ceil_div(n * n, 256)), dim3(256), 0,
+                       ctx->stream, n, (const uint64_t *)d_piv, X);
+    LA_HIP_CHECK(ctx, hipGetLastError());
+    if (n >= TRSM_MIN_N) {
+        if ((st = trsm_blocked(ctx, 'L', n, A, lda, n, X, ldx))) return st;
+        return trsm_blocked(ctx, 'U', n, A, lda, n, X, ldx);
+    }
+    if ((st = trsv_blocked<double>(ctx, 'L', false, true, n, A, lda, n, X, ldx))) return st;
+    return trsv_blocked<double>(ctx, 'U', false, false, n, A, lda, n, X, ldx);
+}
+
+extern "C" int la_dinverse_rm(la_ctx *ctx, int64_t rows, int64_t cols, const double *A,
+                              double *Ainv)
+{
+    if (!ctx) return LA_ERR_ARG;
+    if (rows != cols) return set_error(ctx, LA_ERR_SHAPE, "inverse: not square");  // mod.rs:1035
+    if (rows <= 0) return set_error(ctx, LA_ERR_SHAPE, "inverse: empty matrix");
+    if (!A || !Ainv) return LA_ERR_ARG;
+    const int64_t n = rows;
+    void *base = nullptr;
+    size_t total = rounded(n * n, 8) * 2 + rounded(n, 8) + rounded(4, 4);
+    int st = ws_get(ctx, WS_HOST, total, &base);
+    if (st) return st;
+    HostBufs hb{(char *)base, 0};
+    double *dA = hb.take<double>(n * n), *dX = hb.take<double>(n * n);
+    uint64_t *dpiv = hb.take<uint64_t>(n);
+    int32_t *dinfo = hb.take<int32_t>(4);
+    LA_HIP_CHECK(ctx, hipMemcpyAsync(dA, A, n * n * 8, hipMemcpyHostToDevice, ctx->stream));
+    if ((st = la_dinverse_dev(ctx, n, dA, n, dpiv, dX, n, dinfo))) return st;
+    int32_t info[3];
+    if ((st = read_scratch(ctx, dinfo, info, 3))) return st;
+    if (info[1] != LA_OK) return set_error(ctx, info[1], "getrf: device wait failed");
+    if (info[2] == LA_SINGULAR) return LA_SINGULAR;                 // lu.rs:241-243: None
+    LA_HIP_CHECK(ctx, hipMemcpyAsync(Ainv, dX, n * n * 8, hipMemcpyDeviceToHost, ctx->stream));
     LA_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     return check_sticky(ctx);
 }

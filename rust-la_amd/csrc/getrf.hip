@@ -1243,12 +1243,218 @@ void k_l21_nopiv(T *A, int64_t lda, int rows, int nb, const T *__restrict__ uinv
         if (g * 16 + k < nb) A[(int64_t)(r0 + r) * lda + g * 16 + k] = acc[k];
 }
 
+// ------------------------------------------ tournament pivoting (CALU)
+// For the refinement solver's FP32 factor the pivots of a 64-column panel
+// are chosen by a tournament instead of a search per column (communication-
+// avoiding LU, Grigori, Demmel and Xiang, SIAM J. Matrix Anal. Appl. 32(4),
+// 2011): every node of a reduction tree runs Gaussian elimination with
+// partial pivoting on at most TS_ROWS candidate rows of the panel (their
+// current values) and passes on the rows it chose as pivots, in order; the
+// root's choice becomes the panel's pivot rows.  Those rows are moved to
+// the top of the panel and the panel is then factored without pivoting
+// (k_diag_nopiv, k_l21_nopiv), so the only synchronisation per panel is one
+// launch per tree level instead of one cross-workgroup exchange per column.
+//
+// Node kernel: thread t holds candidate t's row (up to 64 values) in
+// registers, rotated so that the current column is always element 0 (the
+// step's code is the same for every column: no dynamic register index).
+// Per step: wave argmax of (|v[0]|, lowest row), each wave's winning lane
+// writes its row to LDS, one barrier, every thread picks the overall winner
+// and eliminates with it.  LDS slots alternate by step parity, so one
+// barrier per step suffices.
+constexpr int TS_ROWS = 256;
+
+// Maximum of an unsigned value over the wavefront, by DPP row shifts and
+// row broadcasts (no LDS round trip): lane 63 ends with the maximum.
+__device__ inline uint32_t wave_max_u32(uint32_t v)
+{
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false));
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
+__global__ __launch_bounds__(256)
+void k_tsel(const float *__restrict__ P, int64_t ld, int M, int nb,
+            const int *__restrict__ cand_in, int ncand, int *__restrict__ cand_out)
+{
+    __shared__ __align__(16) float prow[2][4][NB_IN];
+    __shared__ uint32_t skey[2][4];
+    __shared__ int sidx[2][4];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int c0 = blockIdx.x * TS_ROWS;
+    const int cnt = min(TS_ROWS, ncand - c0);
+    int row = -1;
+    if (t < cnt) row = cand_in ? cand_in[c0 + t] : c0 + t;
+    bool active = row >= 0 && row < M;
+    float v[NB_IN];
+    {
+        const float *src = P + (int64_t)(active ? row : 0) * ld;
+#pragma unroll
+        for (int c = 0; c < NB_IN; ++c) v[c] = src[min(c, nb - 1)];
+#pragma unroll
+        for (int c = 0; c < NB_IN; ++c)
+            if (c >= nb) v[c] = 0.f;
+    }
+    int *out = cand_out + blockIdx.x * NB_IN;
+    // Columns go in blocks of 16: inside a block the step index s is static,
+    // so the current column is register v[s]; after each block the row is
+    // shifted down by 16 registers (slot c then holds column 16 jb + c).
+    // All 64 steps run (a panel narrower than 64 has zero columns past nb,
+    // whose picks are discarded at the end); the step code has no branch on
+    // the column index.
+#pragma unroll 1
+    for (int jb = 0; jb < NB_IN / 16; ++jb) {
+#pragma unroll
+        for (int s = 0; s < 16; ++s) {
+            const int j = 16 * jb + s, buf = j & 1;
+            // key = bits(|v|) + 2 (a NaN value 1, the lowest), so every
+            // active row has a key >= 1 and the node always chooses
+            // min(nb, rows) pivots; 0 marks "no row left".  Ties go to the
+            // lowest lane, then the lowest wave: a fixed rule (any rule
+            // serves a tournament).
+            const uint32_t ab = __float_as_uint(v[s]) & 0x7fffffffu;
+            const uint32_t key = active ? (ab > 0x7f800000u ? 1u : ab + 2u) : 0u;
+            const uint32_t wkey = wave_max_u32(key);
+            const uint64_t hit = __ballot(key == wkey);
+            const int wlane = __ffsll((unsigned long long)hit) - 1;
+            const int widx = __builtin_amdgcn_readlane(row, wlane);
+            if (lane == wlane && wkey != 0) {
+#pragma unroll
+                for (int c = 4 * (s / 4); c < NB_IN; c += 4)
+                    *reinterpret_cast<float4 *>(&prow[buf][wv][c]) =
+                        make_float4(v[c], v[c + 1], v[c + 2], v[c + 3]);
+            }
+            if (lane == 0) {
+                skey[buf][wv] = wkey;
+                sidx[buf][wv] = widx;
+            }
+            __syncthreads();
+            uint32_t bk = skey[buf][0];
+            int bi = sidx[buf][0], bw = 0;
+#pragma unroll
+            for (int q = 1; q < 4; ++q)
+                if (skey[buf][q] > bk) {
+                    bk = skey[buf][q];
+                    bi = sidx[buf][q];
+                    bw = q;
+                }
+            // bk == 0: no candidate left (fewer rows than columns), the
+            // step records -1 and changes nothing
+            if (t == 0 && j < nb) out[j] = (bk != 0) ? bi : -1;
+            if (active && row == bi) active = false;
+            const float4 *pr4 = reinterpret_cast<const float4 *>(prow[buf][bw]);
+            float pr[NB_IN];
+#pragma unroll
+            for (int c4 = s / 4; c4 < NB_IN / 4; ++c4) {
+                const float4 q = pr4[c4];
+                pr[4 * c4] = q.x;
+                pr[4 * c4 + 1] = q.y;
+                pr[4 * c4 + 2] = q.z;
+                pr[4 * c4 + 3] = q.w;
+            }
+            const float p0 = pr[s];
+            // the multipliers only steer the choice of later pivots: a
+            // reciprocal (within 1 ulp) serves
+            const float rp = (p0 != 0.f) ? __builtin_amdgcn_rcpf(p0) : 0.f;
+            const float l = (active && bk != 0) ? v[s] * rp : 0.f;
+            // eliminate every column right of s (columns past nb are zero)
+#pragma unroll
+            for (int c = s + 1; c < NB_IN; ++c) v[c] = fmaf(-l, pr[c], v[c]);
+        }
+#pragma unroll
+        for (int c = 0; c < NB_IN; ++c) v[c] = (c + 16 < NB_IN) ? v[c + 16] : 0.f;
+    }
+    for (int r = nb + t; r < NB_IN; r += blockDim.x) out[r] = -1;
+}
+
+// Net row movement that brings the winners w[0..nb) (panel-relative rows)
+// to rows 0..nb-1 of the panel: row i receives w[i]; every non-winner row
+// among 0..nb-1 moves to a row vacated by a winner at or below nb (both
+// lists in ascending order).  swapmap: count, then (dst, src) global rows.
+__global__ void k_tsel_moves(int nb, int kglob, const int *__restrict__ w, int64_t *swapmap)
+{
+    __shared__ int s_w[NB_IN];
+    __shared__ int s_win_top[NB_IN];      // s_win_top[i] = 1 if row i < nb is a winner
+    __shared__ int s_disp[NB_IN], s_vac[NB_IN];
+    __shared__ int s_nd, s_nv;
+    const int t = threadIdx.x;
+    if (t < NB_IN) {
+        s_w[t] = t < nb ? w[t] : -1;
+        s_win_top[t] = 0;
+    }
+    if (t == 0) { s_nd = 0; s_nv = 0; }
+    __syncthreads();
+    if (t < nb && s_w[t] >= 0 && s_w[t] < nb) s_win_top[s_w[t]] = 1;
+    __syncthreads();
+    if (t == 0) {
+        int nd = 0, nv = 0;
+        for (int i = 0; i < nb; ++i)
+            if (!s_win_top[i]) s_disp[nd++] = i;
+        for (int i = 0; i < nb; ++i)
+            if (s_w[i] >= nb) s_vac[nv++] = s_w[i];
+        // the vacated rows in ascending order
+        for (int a = 1; a < nv; ++a) {
+            const int x = s_vac[a];
+            int b = a - 1;
+            while (b >= 0 && s_vac[b] > x) { s_vac[b + 1] = s_vac[b]; --b; }
+            s_vac[b + 1] = x;
+        }
+        s_nd = nd;
+        s_nv = nv;
+        int cnt = 0;
+        for (int i = 0; i < nb; ++i)
+            if (s_w[i] != i) {
+                swapmap[1 + 2 * cnt] = (int64_t)kglob + i;
+                swapmap[2 + 2 * cnt] = (int64_t)kglob + s_w[i];
+                ++cnt;
+            }
+        for (int a = 0; a < nd; ++a) {
+            swapmap[1 + 2 * cnt] = (int64_t)kglob + s_vac[a];
+            swapmap[2 + 2 * cnt] = (int64_t)kglob + s_disp[a];
+            ++cnt;
+        }
+        swapmap[0] = cnt;
+    }
+}
+
+// Chooses the panel's pivot rows by the tournament and applies the row
+// movement to every column of A (ld lda, n columns) and to perm.
+static int tournament_pivots(la_ctx *ctx, float *A, int64_t lda, int64_t n, int64_t k, int M,
+                             int nb, int *cand0, int *cand1, int64_t *swapmap, uint64_t *perm)
+{
+    const float *P = A + k * lda + k;
+    int ncand = M;
+    const int *in = nullptr;
+    int *out = cand0;
+    for (;;) {
+        const int nodes = (int)ceil_div(ncand, TS_ROWS);
+        hipLaunchKernelGGL(k_tsel, dim3(nodes), dim3(256), 0, ctx->stream, P, lda, M, nb, in, ncand,
+                           out);
+        LA_HIP_CHECK(ctx, hipGetLastError());
+        if (nodes == 1) break;
+        ncand = nodes * NB_IN;
+        in = out;
+        out = (out == cand0) ? cand1 : cand0;
+    }
+    hipLaunchKernelGGL(k_tsel_moves, dim3(1), dim3(64), 0, ctx->stream, nb, (int)k,
+                       (const int *)out, swapmap);
+    LA_HIP_CHECK(ctx, hipGetLastError());
+    return launch_rowmoves<float>(ctx, A, lda, n, 0, 0, swapmap, perm);
+}
+
 // In-place LU of the m x n matrix A without pivoting (right-looking, the
 // same outer/inner blocking and trailing updates as getrf_core).  *d_tau is
-// the static-pivot threshold; *d_npert counts perturbed pivots.
+// the static-pivot threshold; *d_npert counts perturbed pivots.  With d_perm
+// non-null the pivots of every panel are chosen by the tournament first
+// (rows moved, d_perm permuted alike; d_perm must hold the identity on
+// entry), and the panel is then factored without pivoting.
 template <typename T>
 int getrf_nopiv_core(la_ctx *ctx, int64_t m, int64_t n, T *A, int64_t lda, const float *d_tau,
-                     int32_t *d_npert, int upd_prec)
+                     int32_t *d_npert, int upd_prec, uint64_t *d_perm)
 {
     static_assert(sizeof(T) == 4, "the unpivoted factor is FP32");
     if (m > INT32_MAX / 2 || n > INT32_MAX / 2)
@@ -1257,11 +1463,16 @@ int getrf_nopiv_core(la_ctx *ctx, int64_t m, int64_t n, T *A, int64_t lda, const
     constexpr int NBO = 2 * NB_OUT;               // outer block width (trailing GEMM K)
     constexpr int NLINV = NBO / NB_IN;
     const size_t inv_bytes = (size_t)(NLINV + 1) * NB_IN * NB_IN * sizeof(T);
+    const size_t cand_bytes = ws_align((size_t)ceil_div(m, TS_ROWS) * NB_IN * sizeof(int));
+    const size_t map_bytes = ws_align((size_t)(1 + 4 * NB_IN) * sizeof(int64_t));
     void *base = nullptr;
-    int st = ws_get(ctx, WS_PANEL, inv_bytes, &base);
+    int st = ws_get(ctx, WS_PANEL, ws_align(inv_bytes) + 2 * cand_bytes + map_bytes, &base);
     if (st) return st;
     T *linv_ring = (T *)base;
     T *uinv = linv_ring + (size_t)NLINV * NB_IN * NB_IN;
+    int *cand0 = (int *)((char *)base + ws_align(inv_bytes));
+    int *cand1 = (int *)((char *)cand0 + cand_bytes);
+    int64_t *swapmap = (int64_t *)((char *)cand1 + cand_bytes);
     if ((st = fill_i32(ctx, d_npert, 1, 0))) return st;
 
     for (int64_t K0 = 0; K0 < kmax; K0 += NBO) {
@@ -1270,6 +1481,13 @@ int getrf_nopiv_core(la_ctx *ctx, int64_t m, int64_t n, T *A, int64_t lda, const
             const int nbi = (int)std::min<int64_t>(NB_IN, K0 + nbo - k);
             T *Ap = A + k * lda + k;
             T *linv = linv_ring + (size_t)((k - K0) / NB_IN) * NB_IN * NB_IN;
+            if (d_perm) {
+                ProfScope ps(ctx, LA_K_PIVSEL, (double)(m - k) * nbi * nbi,
+                             (double)(m - k) * nbi * sizeof(T));
+                if ((st = tournament_pivots(ctx, A, lda, n, k, (int)(m - k), nbi, cand0, cand1,
+                                            swapmap, d_perm)))
+                    return st;
+            }
             {
                 ProfScope ps(ctx, LA_K_PANEL, (double)(m - k) * nbi * nbi,
                              2.0 * (m - k) * nbi * sizeof(T));
@@ -1360,7 +1578,7 @@ int noalloc_scratch_init(int device)
     return LA_OK;
 }
 template int getrf_nopiv_core<float>(la_ctx *, int64_t, int64_t, float *, int64_t, const float *,
-                                     int32_t *, int);
+                                     int32_t *, int, uint64_t *);
 
 }  // namespace lahip
 

@@ -53,6 +53,39 @@ __global__ void k_convert_f32(int64_t rows, int64_t cols, const double *A, int64
     }
 }
 
+// A32 = fl32(A) as k_convert_f32, and *amax_bits = max |A32| (as bits)
+__global__ __launch_bounds__(256)
+void k_convert_f32_amax(int64_t n, const double *__restrict__ A, int64_t lda, float *B,
+                        int64_t ldb, unsigned *amax_bits)
+{
+    __shared__ float wmax[4];
+    const int64_t e = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+    float m = 0.f;
+    if (e < n * n) {
+        const int64_t r = e / n, c = e - r * n;
+        if ((n & 3) == 0 && (lda & 1) == 0) {
+            const double2 *src = reinterpret_cast<const double2 *>(A + r * lda + c);
+            const double2 a = src[0], b = src[1];
+            const float4 o = make_float4((float)a.x, (float)a.y, (float)b.x, (float)b.y);
+            *reinterpret_cast<float4 *>(B + r * ldb + c) = o;
+            m = fmaxf(fmaxf(fabsf(o.x), fabsf(o.y)), fmaxf(fabsf(o.z), fabsf(o.w)));
+        } else {
+            for (int q = 0; q < 4 && e + q < n * n; ++q) {
+                const int64_t rr = (e + q) / n, cc = (e + q) - rr * n;
+                const float f = (float)A[rr * lda + cc];
+                B[rr * ldb + cc] = f;
+                m = fmaxf(m, fabsf(f));
+            }
+        }
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        atomicMax(amax_bits, __float_as_uint(fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]))));
+}
+
 // Column maxima of |A| (non-negative doubles compared as bit patterns):
 // grid (column blocks of 256, row chunks of 256).
 __global__ __launch_bounds__(256) void k_colmax(int64_t n, const double *A, int64_t lda,
@@ -406,6 +439,8 @@ struct IrWork {
     int64_t *ipiv = nullptr;
     double *R = nullptr, *D = nullptr, *norms = nullptr;
     int32_t *status = nullptr;
+    // tournament-pivoted (CALU) form: perm / A32 as the pivoted form
+    bool calu = false;
     // randomised-butterfly form
     bool rbt = false;
     int64_t np = 0;                      // padded order (multiple of 4)
@@ -421,8 +456,9 @@ struct IrWork {
     double *partial = nullptr, *hcoef = nullptr;
 };
 
-constexpr int GM_RESTART = 60;                 // Krylov vectors per correction
-constexpr int GM_MAXV = GM_RESTART + 1;
+constexpr int GM_RESTART = 200;                // Krylov vectors per correction
+constexpr int GM_MAXV = GM_RESTART + 1;         // <= 256: one k_dot_reduce thread each
+static_assert(GM_MAXV <= 256, "k_dot_reduce has 256 threads");
 constexpr int GM_DOT_BLOCKS = 128;             // row chunks of the dot kernel
 constexpr double GM_TOL = 1e-6;                // relative reduction per correction
 
@@ -585,7 +621,7 @@ static int basis_dots(la_ctx *ctx, int64_t n, int nv, const IrWork &w, const dou
 {
     hipLaunchKernelGGL(k_dot_partial, dim3(GM_DOT_BLOCKS, nv), dim3(256), 0, ctx->stream, n,
                        w.V, n, x, w.partial);
-    hipLaunchKernelGGL(k_dot_reduce, dim3(1), dim3(64), 0, ctx->stream, nv, w.partial, w.hcoef);
+    hipLaunchKernelGGL(k_dot_reduce, dim3(1), dim3(256), 0, ctx->stream, nv, w.partial, w.hcoef);
     LA_HIP_CHECK(ctx, hipGetLastError());
     LA_HIP_CHECK(ctx, hipMemcpyAsync(h_host, w.hcoef, nv * sizeof(double), hipMemcpyDeviceToHost,
                                      ctx->stream));
@@ -740,6 +776,16 @@ static int ir_attempt(la_ctx *ctx, int64_t n, int64_t nrhs, const double *A, int
         if ((st = fill_i32(ctx, w.status, 1, 0))) return st;
         if ((st = getrf_nopiv_core<float>(ctx, w.np, w.np, w.A32, w.ld32, w.tau, w.npert, prec)))
             return st;
+    } else if (w.calu) {
+        if ((st = fill_i32(ctx, (int32_t *)w.amax, 1, 0))) return st;
+        hipLaunchKernelGGL(k_convert_f32_amax, dim3((unsigned)ceil_div(n * n, 1024)), dim3(256), 0,
+                           ctx->stream, n, A, lda, w.A32, w.ld32, w.amax);
+        hipLaunchKernelGGL(k_rbt_tau, dim3(1), dim3(64), 0, ctx->stream, w.amax, w.tau);
+        LA_HIP_CHECK(ctx, hipGetLastError());
+        if ((st = fill_i32(ctx, w.status, 1, 0))) return st;
+        if ((st = iota_u64(ctx, w.perm, n))) return st;
+        if ((st = getrf_nopiv_core<float>(ctx, n, n, w.A32, w.ld32, w.tau, w.npert, prec, w.perm)))
+            return st;
     } else {
         if (prec == LA_PREC_FP32) {
             hipLaunchKernelGGL(k_convert_f32, dim3((unsigned)ceil_div(n * n, 1024)), dim3(256), 0,
@@ -776,7 +822,8 @@ static int ir_attempt(la_ctx *ctx, int64_t n, int64_t nrhs, const double *A, int
         if (!(eta > tol) || it >= max_iter || !std::isfinite(eta)) break;
         // the unpivoted factor is abandoned for the pivoted one as soon as
         // refinement stagnates (less than a halving per step) or runs long
-        if (w.rbt && ((it >= 2 && eta > 0.5 * eta_prev) || it >= RBT_MAX_STEPS)) break;
+        if ((w.rbt || w.calu) && ((it >= 2 && eta > 0.5 * eta_prev) || it >= RBT_MAX_STEPS))
+            break;
         eta_prev = eta;
         if (w.gmres) {
             if ((st = gmres_correction(ctx, n, nrhs, A, lda, w, w.R, w.D, &report->inner_iterations)))
@@ -818,12 +865,16 @@ extern "C" int la_dgesv_ir_dev(la_ctx *ctx, int64_t n, int64_t nrhs, const doubl
         return LA_ERR_ARG;
     const bool rbt = (prec & LA_IR_RBT) != 0;
     const bool gm = (prec & LA_IR_GMRES) != 0;
-    prec &= ~(LA_IR_RBT | LA_IR_GMRES);
+    const bool calu = (prec & LA_IR_TOURNAMENT) != 0;
+    prec &= ~(LA_IR_RBT | LA_IR_GMRES | LA_IR_TOURNAMENT);
     if (prec != LA_PREC_FP32 && prec != LA_PREC_BF16 && prec != LA_PREC_FP16)
         return set_error(ctx, LA_ERR_ARG, "dgesv_ir: precision not supported");
     if (rbt && (prec != LA_PREC_FP32 || ((n + 3) / 4) > 65535))
         return set_error(ctx, LA_ERR_ARG,
                          "dgesv_ir: the butterfly form needs FP32 and n <= 262140");
+    if (calu && (rbt || prec != LA_PREC_FP32))
+        return set_error(ctx, LA_ERR_ARG,
+                         "dgesv_ir: tournament pivoting needs FP32 and excludes the butterfly form");
     std::memset(report, 0, sizeof(*report));
     report->prec = prec;
     IrWork w;
@@ -831,19 +882,21 @@ extern "C" int la_dgesv_ir_dev(la_ctx *ctx, int64_t n, int64_t nrhs, const doubl
     if (st) return st;
     w.prec = prec;
     w.rbt = rbt;
+    w.calu = calu;
     w.gmres = gm;
     if (gm && (st = gmres_alloc(ctx, n, w))) return st;
-    report->method = rbt ? LA_IR_METHOD_RBT : LA_IR_METHOD_PIVOTED;
+    report->method = rbt ? LA_IR_METHOD_RBT : calu ? LA_IR_METHOD_TOURNAMENT : LA_IR_METHOD_PIVOTED;
     st = ir_attempt(ctx, n, nrhs, A, lda, B, ldb, X, ldx, prec, tol, max_iter, w, report);
-    if (rbt && (st == LA_NOT_CONVERGED || st == LA_SINGULAR)) {
-        // the unpivoted factor did not serve: partial pivoting, same budget
+    if ((rbt || calu) && (st == LA_NOT_CONVERGED || st == LA_SINGULAR)) {
+        // the factor did not serve: partial pivoting, same budget
         int32_t np = 0;
         LA_HIP_CHECK(ctx, hipMemcpy(&np, w.npert, 4, hipMemcpyDeviceToHost));
         report->perturbed_pivots = np;
         w.rbt = false;
-        report->method = LA_IR_METHOD_RBT_THEN_PIVOTED;
+        w.calu = false;
+        report->method = rbt ? LA_IR_METHOD_RBT_THEN_PIVOTED : LA_IR_METHOD_TOURNAMENT_THEN_PIVOTED;
         st = ir_attempt(ctx, n, nrhs, A, lda, B, ldb, X, ldx, prec, tol, max_iter, w, report);
-    } else if (rbt) {
+    } else if (rbt || calu) {
         int32_t np = 0;
         LA_HIP_CHECK(ctx, hipMemcpy(&np, w.npert, 4, hipMemcpyDeviceToHost));
         report->perturbed_pivots = np;

@@ -158,9 +158,11 @@ int getrf_core(la_ctx *ctx, int64_t m, int64_t n, T *A, int64_t lda, int64_t *d_
 // (once per device; the device must be current).
 int noalloc_scratch_init(int device);
 // LU without pivoting (static pivots below *d_tau, counted in *d_npert)
+// With d_perm (identity on entry) the pivots of every 64-column panel are
+// chosen by a tournament (CALU) and applied to A and d_perm first.
 template <typename T>
 int getrf_nopiv_core(la_ctx *ctx, int64_t m, int64_t n, T *A, int64_t lda, const float *d_tau,
-                     int32_t *d_npert, int upd_prec = LA_PREC_FP32);
+                     int32_t *d_npert, int upd_prec = LA_PREC_FP32, uint64_t *d_perm = nullptr);
 int gemm_lp(la_ctx *ctx, int prec, int64_t M, int64_t N, int64_t K, float alpha, const float *A,
             int64_t lda, const float *B, int64_t ldb, float beta, float *C, int64_t ldc);
 // d_info[0] = pospivsign (1 = even number of swaps)
@@ -176,6 +178,13 @@ int trsv_blocked(la_ctx *ctx, char uplo, bool trans, bool unit, int64_t n,
                  const TF *Tm, int64_t ldt, int64_t nrhs, double *X, int64_t ldx);
 int gather_rows(la_ctx *ctx, int64_t m, int64_t nx, const uint64_t *d_piv, const double *B,
                 int64_t ldb, double *X, int64_t ldx);
+// X <- T^-1 X for many right-hand sides: blocked TRSM on the FP64 MFMA GEMM
+// with inverted 64 x 64 diagonal blocks ('L' unit lower / 'U' upper of a
+// packed LU)
+int trsm_blocked(la_ctx *ctx, char uplo, int64_t n, const double *Tm, int64_t ldt, int64_t nrhs,
+                 double *X, int64_t ldx);
+// right-hand-side counts from which the LU solves use trsm_blocked
+constexpr int64_t TRSM_MIN_RHS = 17, TRSM_MIN_N = 65;
 
 int potrf_blocked(la_ctx *ctx, int64_t n, double *A, int64_t lda, int32_t *d_status);
 

@@ -388,6 +388,91 @@ int trsv_blocked(la_ctx *ctx, char uplo, bool trans, bool unit, int64_t n, const
     return set_error(ctx, LA_ERR_ARG, "trsv: unsupported triangle");
 }
 
+// ------------------------------------------- many right-hand sides (TRSM)
+// For nrhs > NR the substitution becomes a blocked TRSM on the FP64 MFMA
+// GEMM: the 64 x 64 diagonal blocks are inverted once (one workgroup each,
+// in parallel), then block by block in dependency order
+//   Y = inv(T_bb) X_b;   X_rest -= T_rest,b Y;   X_b = Y
+// (right-looking: the update of every later block is one GEMM of K = 64).
+template <int MODE, typename TF>
+__global__ __launch_bounds__(256)
+void k_diag_inverses(int n, const TF *__restrict__ Tm, int64_t ldt, double *Dinv)
+{
+    __shared__ double Ts[TB][TB + 1];
+    __shared__ double Xi[TB][TB + 1];
+    __shared__ double part[4][TB];
+    const int b = blockIdx.x, r0 = b * TB, rb = min(TB, n - r0), tid = threadIdx.x;
+    TF dv[TB * TB / 256];
+#pragma unroll
+    for (int k = 0; k < TB * TB / 256; ++k) {
+        const int e = tid + 256 * k, a = e / TB, c = e - a * TB;
+        dv[k] = Tm[(int64_t)(r0 + min(a, rb - 1)) * ldt + r0 + min(c, rb - 1)];
+    }
+#pragma unroll
+    for (int k = 0; k < TB * TB / 256; ++k) {
+        const int e = tid + 256 * k, a = e / TB, c = e - a * TB;
+        Ts[a][c] = (a < rb && c < rb) ? (double)dv[k] : 0.0;
+    }
+    __syncthreads();
+    invert_diag_block<MODE>(Ts, rb, Xi, part);
+    __syncthreads();
+    double *out = Dinv + (size_t)b * TB * TB;
+    for (int e = tid; e < TB * TB; e += blockDim.x) out[e] = Xi[e / TB][e % TB];
+}
+
+__global__ void k_copy_rows(int64_t rows, int64_t cols, const double *src, int64_t lds,
+                            double *dst, int64_t ldd)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= rows * cols) return;
+    const int64_t r = e / cols, c = e - r * cols;
+    dst[r * ldd + c] = src[r * lds + c];
+}
+
+// X (n x nrhs, ld ldx) <- T^-1 X for the LU triangles ('L' unit lower,
+// 'U' upper) of a packed FP64 factor.
+int trsm_blocked(la_ctx *ctx, char uplo, int64_t n, const double *Tm, int64_t ldt, int64_t nrhs,
+                 double *X, int64_t ldx)
+{
+    if (n <= 0 || nrhs <= 0) return LA_OK;
+    const int64_t nblocks = ceil_div(n, TB);
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t dbytes = al((size_t)nblocks * TB * TB * 8), ybytes = al((size_t)TB * nrhs * 8);
+    void *base = nullptr;
+    int st = ws_get(ctx, WS_LP, dbytes + ybytes, &base);
+    if (st) return st;
+    double *Dinv = (double *)base, *Y = (double *)((char *)base + dbytes);
+    const bool lower = uplo == 'L';
+    if (lower)
+        hipLaunchKernelGGL((k_diag_inverses<MODE_LU_FWD, double>), dim3((unsigned)nblocks),
+                           dim3(256), 0, ctx->stream, (int)n, Tm, ldt, Dinv);
+    else
+        hipLaunchKernelGGL((k_diag_inverses<MODE_LU_BWD, double>), dim3((unsigned)nblocks),
+                           dim3(256), 0, ctx->stream, (int)n, Tm, ldt, Dinv);
+    LA_HIP_CHECK(ctx, hipGetLastError());
+    for (int64_t s = 0; s < nblocks; ++s) {
+        const int64_t b = lower ? s : nblocks - 1 - s;
+        const int64_t r0 = b * TB, rb = std::min<int64_t>(TB, n - r0);
+        double *Xb = X + r0 * ldx;
+        if ((st = gemm_rm<double>(ctx, rb, nrhs, rb, 1.0, Dinv + (size_t)b * TB * TB, TB, Xb, ldx,
+                                  0.0, Y, nrhs, false, false)))
+            return st;
+        if (lower && r0 + rb < n) {
+            if ((st = gemm_rm<double>(ctx, n - r0 - rb, nrhs, rb, -1.0, Tm + (r0 + rb) * ldt + r0,
+                                      ldt, Y, nrhs, 1.0, X + (r0 + rb) * ldx, ldx, false, false)))
+                return st;
+        } else if (!lower && r0 > 0) {
+            if ((st = gemm_rm<double>(ctx, r0, nrhs, rb, -1.0, Tm + r0, ldt, Y, nrhs, 1.0, X, ldx,
+                                      false, false)))
+                return st;
+        }
+        hipLaunchKernelGGL(k_copy_rows, dim3((unsigned)ceil_div(rb * nrhs, 256)), dim3(256), 0,
+                           ctx->stream, rb, nrhs, (const double *)Y, nrhs, Xb, ldx);
+        LA_HIP_CHECK(ctx, hipGetLastError());
+    }
+    return LA_OK;
+}
+
 int gather_rows(la_ctx *ctx, int64_t m, int64_t nx, const uint64_t *d_piv, const double *B,
                 int64_t ldb, double *X, int64_t ldx)
 {

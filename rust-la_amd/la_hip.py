@@ -35,10 +35,14 @@ LA_ERR_IO = 11
 PREC_FP64, PREC_FP32, PREC_BF16, PREC_FP16 = 0, 1, 2, 3
 IR_RBT = 0x100      # option bit for gesv_ir_dev: butterfly transform, no pivoting
 IR_GMRES = 0x200    # option bit for gesv_ir_dev: GMRES-preconditioned corrections
+IR_TOURNAMENT = 0x400  # option bit for gesv_ir_dev: tournament-pivoted (CALU) FP32 factor
 IR_METHOD_PIVOTED, IR_METHOD_RBT, IR_METHOD_RBT_THEN_PIVOTED = 0, 1, 2
+IR_METHOD_TOURNAMENT, IR_METHOD_TOURNAMENT_THEN_PIVOTED = 3, 4
+IR_METHOD_NAMES = ["pivoted", "butterfly", "butterfly then pivoted", "tournament",
+                   "tournament then pivoted"]
 
 KERNEL_CLASSES = ["gemm64", "gemm32", "gemm16", "panel", "trsv", "resid", "trsm", "rowmove",
-                  "batch"]
+                  "batch", "pivsel"]
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _u64p = ctypes.POINTER(ctypes.c_uint64)
@@ -151,6 +155,8 @@ _SIGNATURES = {
     "la_gen_spd_batch_dev": [_vp, _i64, _vp, _vp, ctypes.c_uint64, ctypes.c_uint64, _vp, _vp],
     "la_gen_svd_batch_dev": [_vp, _i64, _vp, _vp, _vp, _vp, ctypes.c_double, ctypes.c_int,
                              ctypes.c_uint64, ctypes.c_uint64, _vp, _vp, _vp],
+    "la_dinverse_rm": [_vp, _i64, _i64, _dp, _dp],
+    "la_dinverse_dev": [_vp, _i64, _vp, _i64, _vp, _vp, _i64, _vp],
     "la_autotune": [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p],
     "la_autotune_load": [_vp, ctypes.c_char_p],
 }
@@ -310,6 +316,16 @@ class Context:
             return None
         self._check(st, "dgesv_rm")
         return x
+
+    def inverse(self, a):
+        """Matrix::inverse (mod.rs:1034-1037); None where the reference returns None."""
+        a = _c(a)
+        out = np.empty_like(a)
+        st = self.lib.la_dinverse_rm(self.h, a.shape[0], a.shape[1], _d(a), _d(out))
+        if st == LA_SINGULAR:
+            return None
+        self._check(st, "dinverse_rm")
+        return out
 
     def chol_new(self, a):
         """Returns (L or None, status)."""

@@ -1,0 +1,136 @@
+"""GPU tests at the full sizes of the dense workloads.
+
+* n = 8192 plain FP64 LU + 16-RHS solve (standard normal A and B, seed
+  2024): the permutation vector must be identical to the oracle's (the
+  C restatement of LUDecomposition::new, src/decomp/lu.rs:104-168) recorded
+  in tests/golden/cfg2_n8192.json together with its tie-freeness
+  certificate; the solution is compared with the oracle's on every 32nd row.
+* DGEMM 8192 x 8192 x 8192 (the Mul of src/matrix/mod.rs:957-980) on
+  standard normal operands: 64 fixed rows of C against the oracle's
+  sequential-sum product of those rows.
+* n = 16384 FP32 LU + FP64 refinement on A = U diag(sigma) V^T (64
+  Householder reflectors per side) for kappa in {1e2, 1e4, 1e6}, with the
+  partial-pivoting factor and with the random-butterfly factor: backward
+  error <= 1e-15 n, and forward error against the exact solution x* within
+  the perturbation bound 2 kappa_inf eta / (1 - kappa_inf eta), with
+  kappa_inf <= n kappa (kappa is the 2-norm condition number)."""
+import json
+import os
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+import pytest
+
+import oracle
+import la_hip
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _cfg2_fixture():
+    path = os.path.join(GOLD, "cfg2_n8192.json")
+    with open(path) as f:
+        return json.load(f)
+
+
+def test_cfg2_lu_pivots_and_solution(tctx, torch_cuda):
+    torch = torch_cuda
+    fx = _cfg2_fixture()
+    n, seed = 8192, fx["seed"]
+    assert fx["min_pivot_gap"] > 1e-10, "fixture must certify a tie-free input"
+    print(f"cfg2 LU: n={n} seed={seed}", flush=True)
+    a = oracle.normal(seed, fx["streams"]["A"], (n, n))
+    b = oracle.normal(seed, fx["streams"]["B"], (n, 16))
+    A = torch.from_numpy(a).cuda()
+    B = torch.from_numpy(b).cuda()
+    del a
+    piv = torch.empty(n, dtype=torch.int64, device="cuda")
+    info = torch.zeros(4, dtype=torch.int32, device="cuda")
+    tctx.getrf_dev(A, piv, info)
+    X = torch.empty_like(B)
+    st = torch.zeros(1, dtype=torch.int32, device="cuda")
+    tctx.getrs_dev(A, piv, B, X, st)
+    torch.cuda.synchronize()
+    assert info[1].item() == 0 and st.item() == 0
+    assert bool(info[0].item()) == fx["pospivsign"]
+    p = piv.cpu().numpy().astype(np.uint64)
+    ref = np.array(fx["piv"], dtype=np.uint64)
+    first = np.nonzero(p != ref)[0]
+    assert first.size == 0, f"pivot vector differs from the oracle first at row {first[:5]}"
+    rows = fx["x_rows"]
+    xs = X[rows].cpu().numpy()
+    xr = np.array(fx["x_sample"])
+    err = np.abs(xs - xr).max(axis=0) / np.array(fx["x_col_inf"])
+    print("cfg2 solution rel. difference per column:", err.max(), flush=True)
+    assert err.max() <= 1e-12, err
+
+
+def test_cfg2_dgemm_8192(tctx, torch_cuda):
+    torch = torch_cuda
+    n = 8192
+    print("cfg2 DGEMM 8192^3", flush=True)
+    a = oracle.normal(2025, 0, (n, n))
+    b = oracle.normal(2025, 1, (n, n))
+    A, B = torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()
+    C = torch.empty_like(A)
+    tctx.gemm_dev(A, B, C)
+    torch.cuda.synchronize()
+    rows = np.arange(37, n, n // 64)[:64]
+    assert rows.size == 64
+    c = C[torch.from_numpy(rows).cuda()].cpu().numpy()
+    # the oracle's sequential sums (mod.rs:965-973) for the sampled rows,
+    # eight row blocks in parallel threads (the C oracle releases the GIL)
+    with ThreadPoolExecutor(8) as ex:
+        parts = list(ex.map(lambda r: oracle.mul(a[r], b), np.array_split(rows, 8)))
+    cref = np.vstack(parts)
+    normwise = np.linalg.norm(c - cref) / np.linalg.norm(cref)
+    # elementwise against |A||B| (the scale of each entry's rounding error)
+    absab = np.abs(a[rows]) @ np.abs(b)
+    elem = (np.abs(c - cref) / absab).max()
+    print(f"cfg2 DGEMM normwise {normwise:.2e} elementwise/|A||B| {elem:.2e}", flush=True)
+    assert normwise <= 1e-12
+    assert elem <= n * 2.2e-16
+
+
+@pytest.fixture(scope="module")
+def cfg3_system(tctx, torch_cuda):
+    torch = torch_cuda
+    n = 16384
+    out = {}
+    A = torch.empty((n, n), dtype=torch.float64, device="cuda")
+    b = torch.empty((n, 1), dtype=torch.float64, device="cuda")
+    xs = torch.empty_like(b)
+
+    def make(kappa):
+        if out.get("kappa") != kappa:
+            tctx.gen_svd_matrix_dev(n, kappa, 64, 2025, A, b=b, xstar=xs)
+            out["kappa"] = kappa
+        return A, b, xs
+    return make
+
+
+@pytest.mark.parametrize("kappa", [1e2, 1e4, 1e6])
+@pytest.mark.parametrize("method", ["pivoted", "butterfly"])
+def test_cfg3_fp32_ir_n16384(tctx, torch_cuda, cfg3_system, kappa, method):
+    torch = torch_cuda
+    n = 16384
+    print(f"cfg3 FP32-IR n={n} kappa={kappa:g} {method}", flush=True)
+    A, b, xs = cfg3_system(kappa)
+    X = torch.empty_like(b)
+    tol = 1e-15 * n
+    prec = la_hip.PREC_FP32 | (la_hip.IR_RBT if method == "butterfly" else 0)
+    rep = tctx.gesv_ir_dev(A, b, X, prec, tol, 50)
+    used = ["pivoted", "butterfly", "butterfly then pivoted"][rep.method]
+    print(f"  status={rep.status} method={used} steps={rep.iterations} "
+          f"eta={rep.backward_error:.2e} ms={rep.total_ms:.1f}", flush=True)
+    assert rep.status == la_hip.LA_OK
+    assert rep.backward_error <= tol
+    eta = tctx.backward_error_dev(A, X, b)
+    assert eta <= tol
+    fwd = ((X - xs).abs().max() / xs.abs().max()).item()
+    # ||x - x*||inf / ||x*||inf <= 2 kappa_inf(A) eta / (1 - kappa_inf eta),
+    # and kappa_inf <= n kappa_2 (kappa is the 2-norm condition number)
+    kinf = n * kappa
+    print(f"  forward error {fwd:.2e} = {fwd / (kappa * eta):.1f} kappa_2 eta", flush=True)
+    assert fwd <= 2 * kinf * eta / (1 - kinf * eta), fwd

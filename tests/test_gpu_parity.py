@@ -301,3 +301,44 @@ def test_matrix_solve_wide_status_order(ctx):
     with pytest.raises(la_hip.LaError) as e:
         ctx.solve(a_ok, b)
     assert e.value.status == la_hip.LA_ERR_SHAPE
+
+
+# ------------------------------------------------ inverse (many RHS, TRSM)
+@pytest.mark.parametrize("n", [512, 1024])
+def test_inverse_vs_oracle(ctx, n):
+    """Matrix::inverse = LUDecomposition::new(a).solve(id) (mod.rs:1034-1037):
+    an n-column solve, which runs as the blocked MFMA TRSM; the result is
+    held to the oracle's inverse at 1e-12 normwise and A * inv(A) = I."""
+    a = oracle.uniform(77, n, (n, n))
+    inv = ctx.inverse(a)
+    assert np.array_equal(inv, ctx.solve(a, np.eye(n)))
+    lu_o, piv_o, _ = oracle.lu_new(a)
+    inv_o = oracle.lu_solve(lu_o, piv_o, np.eye(n))
+    err = rel_fro(inv, inv_o)
+    print(f"inverse n={n}: rel. difference from the oracle {err:.2e}", flush=True)
+    assert err <= 1e-12
+    resid = np.abs(a @ inv - np.eye(n)).max()
+    assert resid <= n * 1e-13
+
+
+def test_many_rhs_solve_vs_oracle(ctx):
+    """A 40-column solve (TRSM path) against the oracle's substitution."""
+    n = 300
+    a = oracle.uniform(78, 0, (n, n))
+    b = oracle.uniform(78, 1, (n, 40))
+    x = ctx.solve(a, b)
+    lu_o, piv_o, _ = oracle.lu_new(a)
+    x_o = oracle.lu_solve(lu_o, piv_o, b)
+    assert rel_inf(x, x_o) <= 1e-12
+
+
+def test_inverse_status(ctx):
+    """mod.rs:1035 asserts square; a singular factor is None (mod.rs:1542)."""
+    with pytest.raises(la_hip.LaError) as e:
+        ctx.inverse(np.ones((2, 3)))
+    assert e.value.status == la_hip.LA_ERR_SHAPE
+    assert ctx.inverse(np.array([[2.0, 6.0], [1.0, 3.0]])) is None
+    inv = ctx.inverse(np.array([[6.0, -7.0, 10.0], [0.0, 3.0, -1.0], [0.0, 5.0, -7.0]]))
+    inv_o = oracle.lu_solve(*oracle.lu_new(np.array([[6.0, -7.0, 10.0], [0.0, 3.0, -1.0],
+                                                     [0.0, 5.0, -7.0]]))[:2], np.eye(3))
+    assert np.array_equal(inv, inv_o)

@@ -336,3 +336,50 @@ def test_oracle_reproduces_cfg1_fixture():
     assert piv.tolist() == g["piv"] and sign == g["pospivsign"]
     assert oracle.lu_solve(lu, piv, b).ravel().tolist() == g["x"]
     assert oracle.lu_det(lu, sign) == g["det"]
+
+
+# ------------------------------------------------ f32 surface (simd.rs)
+def test_oracle_smul_exact_on_integers():
+    """simd.rs:189-219 on small-integer matrices: every product and partial
+    sum is exact in FP32, so the result equals the exact product; shapes
+    with p mod 4 != 0 exercise the scalar tail columns."""
+    rng = np.random.default_rng(3)
+    for (m, k, p) in [(3, 4, 5), (1, 1, 1), (7, 9, 4), (5, 3, 11)]:
+        a = rng.integers(-20, 21, size=(m, k)).astype(np.float32)
+        b = rng.integers(-20, 21, size=(k, p)).astype(np.float32)
+        assert np.array_equal(oracle.smul(a, b), (a.astype(np.float64) @ b).astype(np.float32))
+
+
+def test_oracle_smul_rounding_bound_and_shape():
+    rng = np.random.default_rng(4)
+    a = rng.standard_normal((33, 70)).astype(np.float32)
+    b = rng.standard_normal((70, 21)).astype(np.float32)
+    c = oracle.smul(a, b).astype(np.float64)
+    exact = a.astype(np.float64) @ b.astype(np.float64)
+    bound = 70 * 2.0 ** -24 * (np.abs(a).astype(np.float64) @ np.abs(b))
+    assert (np.abs(c - exact) <= bound).all()
+    # a transposed operand or a dropped term would break the bound
+    assert not np.allclose(c, (a.astype(np.float64) @ b[::-1]), atol=1e-3)
+    with pytest.raises(oracle.OracleError):
+        oracle.smul(a, a)
+
+
+def test_oracle_sdot():
+    x = np.arange(7, dtype=np.float32)
+    assert oracle.sdot(x, x) == 91.0                       # 0^2 + ... + 6^2, tail included
+    assert oracle.sdot(x[:3], x[:3]) == 5.0                # shorter than one lane group
+    rng = np.random.default_rng(5)
+    a, b = rng.standard_normal(1001).astype(np.float32), rng.standard_normal(1001).astype(np.float32)
+    exact = float(a.astype(np.float64) @ b.astype(np.float64))
+    assert abs(oracle.sdot(a, b) - exact) <= 1001 * 2.0 ** -24 * float(np.abs(a) @ np.abs(b))
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (2, 3), (3, 2), (64, 64), (300, 300), (257, 130)])
+def test_oracle_lu_colcopy_bit_identical(shape):
+    """The contiguous-column variant used to write the n = 8192 fixture does
+    the same operations in the same order as the statement of lu.rs:104-168."""
+    a = oracle.normal(11, shape[0], shape)
+    r1 = oracle.lu_new(a, want_gap=True)
+    r2 = oracle.lu_new_colcopy(a, want_gap=True)
+    assert np.array_equal(r1[0], r2[0]) and np.array_equal(r1[1], r2[1])
+    assert r1[2] == r2[2] and r1[3] == r2[3]
