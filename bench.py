@@ -152,6 +152,7 @@ def batched_lu_ir(ctx, steps, warmup, count=4096, seed=4096):
     Xs = torch.empty_like(B)
     ctx.gen_svd_batch_dev(dn, dr, dOA, dOB, 1e3, 64, 11, 0, A, B, Xs)
     flop = float((2.0 / 3.0 * n64 ** 3).sum())
+    ctx.autotune(1, 32, 1024)
     res = {"count": count, "n_range": [32, 1024], "kappa": 1e3, "fp64_equiv_flop": flop}
     X16 = None
     for name, prec in (("fp16_ir", la_hip.PREC_FP16), ("fp32_ir", la_hip.PREC_FP32),
@@ -239,6 +240,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--seeds", type=int, default=len(SEEDS),
+                    help="systems the timed steps cycle over (profiling runs use fewer)")
+    ap.add_argument("--no-others", action="store_true",
+                    help="skip the other FP32 factors (profiling runs)")
+    ap.add_argument("--no-cfg2", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}): the solver and its "
@@ -259,7 +265,8 @@ def main():
         # ---------------- the five systems of the headline, resident in HBM
         systems = []
         t_gen = []
-        for s in SEEDS:
+        used_seeds = SEEDS[:max(1, args.seeds)]
+        for s in used_seeds:
             A = torch.empty((n, n), dtype=torch.float64, device="cuda")
             b = torch.empty((n, 1), dtype=torch.float64, device="cuda")
             xs = torch.empty_like(b)
@@ -300,7 +307,8 @@ def main():
                             "forward_error_vs_xstar": ((X - xs).abs().max()
                                                        / xs.abs().max()).item()})
             return res
-        others = {m: per_system(la_hip.PREC_FP32 | o) for m, o in opt.items() if m != args.method}
+        others = {} if args.no_others else {
+            m: per_system(la_hip.PREC_FP32 | o) for m, o in opt.items() if m != args.method}
         W = torch.empty((n, n), dtype=torch.float64, device="cuda")
         piv = torch.empty(n, dtype=torch.int64, device="cuda")
         info = torch.zeros(4, dtype=torch.int32, device="cuda")
@@ -353,11 +361,11 @@ def main():
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": "fp32-factor+fp64-refine",
             "data": "synthetic: seeded Householder-product A = U diag(sigma) V^T, normal b; "
-                    f"steps cycle over seeds {SEEDS}",
+                    f"steps cycle over seeds {used_seeds}",
             "config": {"workload": f"dense solve n={n}, kappa={kappa:g}, 1 RHS, FP32 LU "
                                    f"({ {'rbt': 'random butterfly, no pivoting', 'pivoted': 'partial pivoting', 'tournament': 'tournament pivoting'}[args.method]})"
                                    " + FP64 IR to eta<=1e-15*n",
-                       "n": n, "kappa": kappa, "nrhs": 1, "seeds": SEEDS},
+                       "n": n, "kappa": kappa, "nrhs": 1, "seeds": used_seeds},
             "headline_steps": steps,
             "all_steps_converged": all_ok,
             "fp32_ir_other_factors": {
@@ -415,7 +423,8 @@ def main():
                                                                / xs.abs().max()).item(),
                                     "tflops": flop / (ms * 1e-3) / 1e12}
             del A
-        out["cfg2_n8192"] = cfg2(ctx, torch, 2)
+        if not args.no_cfg2:
+            out["cfg2_n8192"] = cfg2(ctx, torch, 2)
         if not args.no_batched:
             out["batched_cholesky"] = batched_cholesky(ctx, args.steps, args.warmup)
             out["batched_lu_ir"] = batched_lu_ir(ctx, args.steps, args.warmup)

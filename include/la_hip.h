@@ -317,11 +317,15 @@ int la_dgesv_ir_batched_dev(la_ctx *ctx, la_batch_plan *plan,
                             double tol, int max_iter, int32_t *d_status,
                             int32_t *d_iters, double *d_berr);
 
-/* Run the tile autotuner for `kind` over the size buckets of [nmin, nmax]
- * and store the winners in `cache_path` (JSON); later plans read it.
- * cache_path may be NULL to tune in memory only. */
+/* Run the autotuner for `kind` and store the winners in `cache_path`
+ * (JSON); later plans of this context use them.  kind 0 (Cholesky, orders
+ * in [nmin, nmax] <= 256): panel width and MFMA row tiles per size bucket.
+ * kind 1 (LU + refinement, orders in [nmin, nmax] <= 1024): workgroups per
+ * CU of the persistent launch for each working precision (FP64, FP32,
+ * FP16), timed on a synthetic batch of 4096 kappa = 1e3 systems.  cache_path may be NULL
+ * to tune in memory only. */
 int la_autotune(la_ctx *ctx, int kind, int nmin, int nmax, const char *cache_path);
-/* Load a previously written autotune cache. */
+/* Load a previously written autotune cache (either kind). */
 int la_autotune_load(la_ctx *ctx, const char *cache_path);
 
 /* ------------------------------------------------------- test generators

@@ -7,6 +7,7 @@
 // buffers replay the graph.  The launch is a single persistent kernel; the
 // queue is reset by the kernel itself, so the graph has no memset node.
 #include "la_common.hpp"
+#include <cmath>
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
@@ -66,8 +67,9 @@ static void describe(la_batch_plan *p) {
     if (p->kind == 1) {
         static const char *pn[] = {"fp64", "fp32", "bf16", "fp16"};
         snprintf(buf, sizeof buf,
-                 "kind=lu-ir count=%lld grid=%d working precision=%s panel width=%d",
-                 (long long)p->count, p->grid, pn[p->prec], p->prec == LA_PREC_FP64 ? 16 : 32);
+                 "kind=lu-ir count=%lld grid=%d working precision=%s panel width=%d%s",
+                 (long long)p->count, p->grid, pn[p->prec], p->prec == LA_PREC_FP64 ? 16 : 32,
+                 p->ctx->lu_tuned ? " (grid autotuned)" : " (grid at the occupancy limit)");
         p->desc = buf;
         return;
     }
@@ -134,6 +136,8 @@ static int create_lu_ir_plan(la_ctx *ctx, int64_t count, const int32_t *h_n,
     if ((e = hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking)) != hipSuccess)
         return fail(e, "plan stream");
     p->grid = lu_ir_batched_grid(ctx, prec);
+    if (ctx->lu_tuned && ctx->lu_wg_per_cu[prec] > 0)
+        p->grid = std::min(p->grid, std::max(1, ctx->num_cus) * ctx->lu_wg_per_cu[prec]);
     describe(p);
     *out = p;
     return LA_OK;
@@ -271,10 +275,110 @@ extern "C" int la_dgesv_ir_batched_dev(la_ctx *ctx, la_batch_plan *p, const int6
 }
 
 // ---------------------------------------------------------------- autotune
-// For each size bucket, time every candidate panel width on a synthetic
-// batch of that bucket's sizes and keep the fastest.
+// Kind 1 (batched LU + refinement): for each working precision, time the
+// persistent kernel at 1, 2 and the occupancy limit of workgroups per CU on
+// a synthetic batch of 4096 systems (orders log-uniform in [nmin, nmax],
+// kappa = 1e3, 1-8 right-hand sides) and keep the fastest.  The batch size
+// matters: few large problems favour one workgroup per CU (the largest
+// problem's latency sets the time), many favour more (throughput).
+static int autotune_lu(la_ctx *ctx, int nmin, int nmax, const char *cache_path)
+{
+    const int count = 4096;
+    std::vector<int32_t> hn(count), hr(count);
+    std::vector<int64_t> ha(count), hb(count);
+    uint64_t s = 0x13198A2E03707344ull;
+    int64_t oa = 0, ob = 0;
+    const double lr = std::log((double)nmax / nmin);
+    for (int i = 0; i < count; ++i) {
+        s ^= s << 13; s ^= s >> 7; s ^= s << 17;
+        const double u = (double)(s >> 11) * 0x1.0p-53;
+        hn[i] = std::max(nmin, std::min(nmax, (int)std::lround(nmin * std::exp(u * lr))));
+        hr[i] = 1 + (int)((s >> 3) % 8);
+        ha[i] = oa; hb[i] = ob;
+        oa += (int64_t)hn[i] * hn[i];
+        ob += (int64_t)hn[i] * hr[i];
+    }
+    double *dA = nullptr, *dB = nullptr, *dX = nullptr, *dXs = nullptr, *dE = nullptr;
+    int64_t *dOA = nullptr, *dOB = nullptr;
+    int32_t *dN = nullptr, *dR = nullptr, *dS = nullptr, *dI = nullptr;
+    hipError_t he = hipMalloc(&dA, oa * 8);
+    if (he == hipSuccess) he = hipMalloc(&dB, ob * 8);
+    if (he == hipSuccess) he = hipMalloc(&dX, ob * 8);
+    if (he == hipSuccess) he = hipMalloc(&dXs, ob * 8);
+    if (he == hipSuccess) he = hipMalloc(&dE, count * 8);
+    if (he == hipSuccess) he = hipMalloc(&dOA, count * 8);
+    if (he == hipSuccess) he = hipMalloc(&dOB, count * 8);
+    if (he == hipSuccess) he = hipMalloc(&dN, count * 4);
+    if (he == hipSuccess) he = hipMalloc(&dR, count * 4);
+    if (he == hipSuccess) he = hipMalloc(&dS, count * 4);
+    if (he == hipSuccess) he = hipMalloc(&dI, count * 4);
+    if (he == hipSuccess) he = hipMemcpy(dOA, ha.data(), count * 8, hipMemcpyHostToDevice);
+    if (he == hipSuccess) he = hipMemcpy(dOB, hb.data(), count * 8, hipMemcpyHostToDevice);
+    if (he == hipSuccess) he = hipMemcpy(dN, hn.data(), count * 4, hipMemcpyHostToDevice);
+    if (he == hipSuccess) he = hipMemcpy(dR, hr.data(), count * 4, hipMemcpyHostToDevice);
+    int st = he == hipSuccess ? LA_OK : hip_fail(ctx, he, "autotune alloc");
+    if (st == LA_OK)
+        st = la_gen_svd_batch_dev(ctx, count, dN, dR, dOA, dOB, 1e3, 16, 3, 0, dA, dB, dXs);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (st == LA_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+        st = set_error(ctx, LA_ERR_HIP, "autotune events");
+    int best[4] = {0, 0, 0, 0};
+    float best_ms[4] = {0, 0, 0, 0};
+    const int precs[3] = {LA_PREC_FP64, LA_PREC_FP32, LA_PREC_FP16};
+    for (int pi = 0; pi < 3 && st == LA_OK; ++pi) {
+        const int prec = precs[pi];
+        la_batch_plan *p = nullptr;
+        if ((st = la_batch_plan_create(ctx, 1, count, hn.data(), hr.data(), prec, &p))) break;
+        const int cus = std::max(1, ctx->num_cus);
+        const int maxg = lu_ir_batched_grid(ctx, prec) / cus;
+        float bestms = 1e30f;
+        for (int g : {1, 2, maxg}) {
+            if (g < 1 || g > maxg || st) continue;
+            float ms = 0.f;
+            for (int rep = 0; rep < 3 && st == LA_OK; ++rep) {
+                (void)hipEventRecord(e0, ctx->stream);
+                st = launch_lu_ir_batched(ctx, ctx->stream, prec, g * cus, count, p->d_order,
+                                          p->d_n, p->d_nrhs, dOA, dOB, p->d_offW, p->d_offN, dA,
+                                          dB, dX, p->d_W, p->d_perm, 1e-14, 30, dS, dI, dE,
+                                          p->d_queue);
+                (void)hipEventRecord(e1, ctx->stream);
+                (void)hipEventSynchronize(e1);
+                float t = 0.f;
+                (void)hipEventElapsedTime(&t, e0, e1);
+                if (rep > 0) ms += t;
+            }
+            if (st == LA_OK && ms < bestms) { bestms = ms; best[prec] = g; best_ms[prec] = ms / 2; }
+        }
+        la_batch_plan_destroy(p);
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(dA); (void)hipFree(dB); (void)hipFree(dX); (void)hipFree(dXs); (void)hipFree(dE);
+    (void)hipFree(dOA); (void)hipFree(dOB); (void)hipFree(dN); (void)hipFree(dR); (void)hipFree(dS);
+    (void)hipFree(dI);
+    if (st) return st;
+    memcpy(ctx->lu_wg_per_cu, best, sizeof best);
+    ctx->lu_tuned = true;
+    if (cache_path) {
+        FILE *f = fopen(cache_path, "w");
+        if (!f) return set_error(ctx, LA_ERR_ARG, "autotune: cannot write cache");
+        fprintf(f,
+                "{\"kind\": \"lu-ir\", \"orders\": [%d, %d], "
+                "\"workgroups_per_cu\": [%d, %d, %d], \"ms_per_4096\": [%.3f, %.3f, %.3f]}\n",
+                nmin, nmax, best[LA_PREC_FP64], best[LA_PREC_FP32], best[LA_PREC_FP16],
+                best_ms[LA_PREC_FP64], best_ms[LA_PREC_FP32], best_ms[LA_PREC_FP16]);
+        fclose(f);
+    }
+    return LA_OK;
+}
+
+// Kind 0 (batched Cholesky): for each size bucket, time every candidate
+// panel width on a synthetic batch of that bucket's sizes and keep the
+// fastest.
 extern "C" int la_autotune(la_ctx *ctx, int kind, int nmin, int nmax, const char *cache_path)
 {
+    if (ctx && kind == 1 && nmin >= 1 && nmax <= 1024 && nmin <= nmax)
+        return autotune_lu(ctx, nmin, nmax, cache_path);
     if (!ctx || kind != 0 || nmin < 1 || nmax > 256 || nmin > nmax) return LA_ERR_ARG;
     const int lo[4] = {1, 33, 65, 129}, hi[4] = {32, 64, 128, 256};
     int best[4];
@@ -363,6 +467,20 @@ extern "C" int la_autotune_load(la_ctx *ctx, const char *cache_path)
     size_t nread = fread(buf, 1, sizeof buf - 1, f);
     fclose(f);
     buf[nread] = 0;
+    if (strstr(buf, "\"lu-ir\"")) {
+        const char *kw = strstr(buf, "\"workgroups_per_cu\"");
+        int g[3];
+        if (!kw || !strchr(kw, '[') ||
+            sscanf(strchr(kw, '['), "[%d, %d, %d]", &g[0], &g[1], &g[2]) != 3)
+            return set_error(ctx, LA_ERR_ARG, "autotune: malformed cache");
+        for (int i = 0; i < 3; ++i)
+            if (g[i] < 0 || g[i] > 64) return set_error(ctx, LA_ERR_ARG, "autotune: bad grid in cache");
+        ctx->lu_wg_per_cu[LA_PREC_FP64] = g[0];
+        ctx->lu_wg_per_cu[LA_PREC_FP32] = g[1];
+        ctx->lu_wg_per_cu[LA_PREC_FP16] = g[2];
+        ctx->lu_tuned = true;
+        return LA_OK;
+    }
     const char *k = strstr(buf, "\"panel_width\"");
     int v[4], t[4] = {2, 2, 2, 2};
     if (!k || !strchr(k, '[') ||

@@ -44,15 +44,33 @@ __device__ inline bool better(uint64_t k1, int i1, uint64_t k0, int i0) {
     return k1 > k0 || (k1 == k0 && i1 < i0);
 }
 
+// Maximum of an unsigned value over the wavefront, by DPP row shifts and
+// row broadcasts (no LDS round trip): lane 63 ends with the maximum, which
+// every lane receives.
+__device__ inline uint32_t wave_max_u32(uint32_t v)
+{
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false));
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
+// Wavefront argmax under better(): largest 64-bit key, lowest index among
+// equal keys; every lane returns the winner.  Three DPP maxima (high key
+// word, low key word among the lanes at the high maximum, complemented
+// index among the lanes at the full maximum) instead of a shuffle tree.
 __device__ inline void wave_argmax(uint64_t &key, int &idx) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        uint32_t lo = __shfl_xor((uint32_t)key, off);
-        uint32_t hi = __shfl_xor((uint32_t)(key >> 32), off);
-        int oi = __shfl_xor(idx, off);
-        uint64_t ok = ((uint64_t)hi << 32) | lo;
-        if (better(ok, oi, key, idx)) { key = ok; idx = oi; }
-    }
+    const uint32_t hi = (uint32_t)(key >> 32), lo = (uint32_t)key;
+    const uint32_t mhi = wave_max_u32(hi);
+    const bool at_hi = hi == mhi;
+    const uint32_t mlo = wave_max_u32(at_hi ? lo : 0u);
+    const bool at_max = at_hi && lo == mlo;
+    const uint32_t nidx = wave_max_u32(at_max ? ~(uint32_t)idx : 0u);
+    key = ((uint64_t)mhi << 32) | mlo;
+    idx = (int)~nidx;
 }
 
 // Workgroup argmax; every thread returns the winner.  Uses two barriers.
@@ -1264,19 +1282,6 @@ void k_l21_nopiv(T *A, int64_t lda, int rows, int nb, const T *__restrict__ uinv
 // barrier per step suffices.
 constexpr int TS_ROWS = 256;
 
-// Maximum of an unsigned value over the wavefront, by DPP row shifts and
-// row broadcasts (no LDS round trip): lane 63 ends with the maximum.
-__device__ inline uint32_t wave_max_u32(uint32_t v)
-{
-    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false));
-    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false));
-    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false));
-    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false));
-    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false));
-    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false));
-    return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
-}
-
 __global__ __launch_bounds__(256)
 void k_tsel(const float *__restrict__ P, int64_t ld, int M, int nb,
             const int *__restrict__ cand_in, int ncand, int *__restrict__ cand_out)
@@ -1373,52 +1378,44 @@ void k_tsel(const float *__restrict__ P, int64_t ld, int M, int nb,
 
 // Net row movement that brings the winners w[0..nb) (panel-relative rows)
 // to rows 0..nb-1 of the panel: row i receives w[i]; every non-winner row
-// among 0..nb-1 moves to a row vacated by a winner at or below nb (both
-// lists in ascending order).  swapmap: count, then (dst, src) global rows.
-__global__ void k_tsel_moves(int nb, int kglob, const int *__restrict__ w, int64_t *swapmap)
+// among 0..nb-1 moves to a row vacated by a winner at or below nb, both
+// lists in ascending order.  One wavefront, ranks by ballots.  swapmap:
+// count, then (dst, src) global rows.
+__global__ __launch_bounds__(64)
+void k_tsel_moves(int nb, int kglob, const int *__restrict__ w, int64_t *swapmap)
 {
     __shared__ int s_w[NB_IN];
-    __shared__ int s_win_top[NB_IN];      // s_win_top[i] = 1 if row i < nb is a winner
-    __shared__ int s_disp[NB_IN], s_vac[NB_IN];
-    __shared__ int s_nd, s_nv;
     const int t = threadIdx.x;
-    if (t < NB_IN) {
-        s_w[t] = t < nb ? w[t] : -1;
-        s_win_top[t] = 0;
-    }
-    if (t == 0) { s_nd = 0; s_nv = 0; }
+    const uint64_t below = (t == 0) ? 0ull : (~0ull >> (64 - t));    // lanes < t
+    const int wt = t < nb ? w[t] : -1;
+    s_w[t] = wt;
     __syncthreads();
-    if (t < nb && s_w[t] >= 0 && s_w[t] < nb) s_win_top[s_w[t]] = 1;
-    __syncthreads();
-    if (t == 0) {
-        int nd = 0, nv = 0;
-        for (int i = 0; i < nb; ++i)
-            if (!s_win_top[i]) s_disp[nd++] = i;
-        for (int i = 0; i < nb; ++i)
-            if (s_w[i] >= nb) s_vac[nv++] = s_w[i];
-        // the vacated rows in ascending order
-        for (int a = 1; a < nv; ++a) {
-            const int x = s_vac[a];
-            int b = a - 1;
-            while (b >= 0 && s_vac[b] > x) { s_vac[b + 1] = s_vac[b]; --b; }
-            s_vac[b + 1] = x;
-        }
-        s_nd = nd;
-        s_nv = nv;
-        int cnt = 0;
-        for (int i = 0; i < nb; ++i)
-            if (s_w[i] != i) {
-                swapmap[1 + 2 * cnt] = (int64_t)kglob + i;
-                swapmap[2 + 2 * cnt] = (int64_t)kglob + s_w[i];
-                ++cnt;
-            }
-        for (int a = 0; a < nd; ++a) {
-            swapmap[1 + 2 * cnt] = (int64_t)kglob + s_vac[a];
-            swapmap[2 + 2 * cnt] = (int64_t)kglob + s_disp[a];
-            ++cnt;
-        }
-        swapmap[0] = cnt;
+    // winners already among the top nb rows
+    bool top_is_winner = false;
+    for (int i = 0; i < nb; ++i) top_is_winner |= (s_w[i] == t);
+    const bool displaced = t < nb && !top_is_winner;           // row t leaves
+    const bool vacates = t < nb && wt >= nb;                    // winner wt leaves a hole
+    const bool moves = t < nb && wt != t;
+    const uint64_t bm = __ballot(moves), bd = __ballot(displaced);
+    const int nmoves = __popcll(bm);
+    if (moves) {
+        const int o = __popcll(bm & below);
+        swapmap[1 + 2 * o] = (int64_t)kglob + t;
+        swapmap[2 + 2 * o] = (int64_t)kglob + wt;
     }
+    // the vacated rows in ascending order: rank of wt among them
+    int vrank = 0;
+    if (vacates)
+        for (int i = 0; i < nb; ++i) vrank += (s_w[i] >= nb && s_w[i] < wt);
+    __shared__ int s_vac[NB_IN];
+    if (vacates) s_vac[vrank] = wt;
+    __syncthreads();
+    if (displaced) {
+        const int a = __popcll(bd & below);                    // rank among displaced rows
+        swapmap[1 + 2 * (nmoves + a)] = (int64_t)kglob + s_vac[a];
+        swapmap[2 + 2 * (nmoves + a)] = (int64_t)kglob + t;
+    }
+    if (t == 0) swapmap[0] = nmoves + __popcll(bd);
 }
 
 // Chooses the panel's pivot rows by the tournament and applies the row

@@ -58,19 +58,22 @@ __global__ __launch_bounds__(256)
 void k_convert_f32_amax(int64_t n, const double *__restrict__ A, int64_t lda, float *B,
                         int64_t ldb, unsigned *amax_bits)
 {
+    // grid-stride over groups of four elements, so that the grid (and the
+    // number of atomics on the one maximum word) stays small
     __shared__ float wmax[4];
-    const int64_t e = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
     float m = 0.f;
-    if (e < n * n) {
+    const int64_t total = n * n;
+    for (int64_t e = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4; e < total;
+         e += (int64_t)gridDim.x * blockDim.x * 4) {
         const int64_t r = e / n, c = e - r * n;
         if ((n & 3) == 0 && (lda & 1) == 0) {
             const double2 *src = reinterpret_cast<const double2 *>(A + r * lda + c);
             const double2 a = src[0], b = src[1];
             const float4 o = make_float4((float)a.x, (float)a.y, (float)b.x, (float)b.y);
             *reinterpret_cast<float4 *>(B + r * ldb + c) = o;
-            m = fmaxf(fmaxf(fabsf(o.x), fabsf(o.y)), fmaxf(fabsf(o.z), fabsf(o.w)));
+            m = fmaxf(m, fmaxf(fmaxf(fabsf(o.x), fabsf(o.y)), fmaxf(fabsf(o.z), fabsf(o.w))));
         } else {
-            for (int q = 0; q < 4 && e + q < n * n; ++q) {
+            for (int q = 0; q < 4 && e + q < total; ++q) {
                 const int64_t rr = (e + q) / n, cc = (e + q) - rr * n;
                 const float f = (float)A[rr * lda + cc];
                 B[rr * ldb + cc] = f;
@@ -778,8 +781,9 @@ static int ir_attempt(la_ctx *ctx, int64_t n, int64_t nrhs, const double *A, int
             return st;
     } else if (w.calu) {
         if ((st = fill_i32(ctx, (int32_t *)w.amax, 1, 0))) return st;
-        hipLaunchKernelGGL(k_convert_f32_amax, dim3((unsigned)ceil_div(n * n, 1024)), dim3(256), 0,
-                           ctx->stream, n, A, lda, w.A32, w.ld32, w.amax);
+        hipLaunchKernelGGL(k_convert_f32_amax,
+                           dim3((unsigned)std::min<int64_t>(ceil_div(n * n, 1024), 4096)),
+                           dim3(256), 0, ctx->stream, n, A, lda, w.A32, w.ld32, w.amax);
         hipLaunchKernelGGL(k_rbt_tau, dim3(1), dim3(64), 0, ctx->stream, w.amax, w.tau);
         LA_HIP_CHECK(ctx, hipGetLastError());
         if ((st = fill_i32(ctx, w.status, 1, 0))) return st;

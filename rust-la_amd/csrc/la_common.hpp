@@ -80,6 +80,10 @@ struct la_ctx {
     // batched Cholesky tile configuration per size bucket: panel width | row tiles << 8
     int chol_nb[4] = {16 | (2 << 8), 16 | (2 << 8), 16 | (2 << 8), 16 | (2 << 8)};
     bool chol_tuned = false;
+    // batched LU + refinement: workgroups per CU by working precision
+    // (LA_PREC_FP64 / FP32 / FP16 index), 0 = occupancy limit
+    int lu_wg_per_cu[4] = {0, 0, 0, 0};
+    bool lu_tuned = false;
     bool prof = false;
     std::vector<hipEvent_t> ev_pool;
     std::vector<lahip::ProfRecord> prof_pending;

@@ -184,3 +184,35 @@ def test_full_batch_fp16_vs_fp64(tctx, torch_cuda):
         e16 = np.abs(x16[sl] - xs[sl]).max() / np.abs(xs[sl]).max()
         e64 = np.abs(x64[sl] - xs[sl]).max() / np.abs(xs[sl]).max()
         assert e16 <= 1e-10 and e64 <= 1e-10, (i, n, e16, e64)
+
+
+def test_autotune_lu_grid(tctx, torch_cuda, tmp_path):
+    """la_autotune kind 1: the grid (workgroups per CU) chosen per working
+    precision is written to the JSON cache, loaded back by another context,
+    reported by new plans, and a batch solved with the tuned grid is the same
+    bit for bit as with the default grid (the grid only changes which
+    workgroup takes which problem)."""
+    import json
+    torch = torch_cuda
+    rng = np.random.default_rng(8)
+    sizes = np.round(32 * 8 ** rng.random(40)).astype(np.int32)
+    nrhs = rng.integers(1, 9, size=len(sizes)).astype(np.int32)
+    bt = make_batch(torch, tctx, sizes, nrhs, nrefl=16)
+    before = solve_batch(torch, tctx, bt, la_hip.PREC_FP32)[0]
+    path = str(tmp_path / "lu_tune.json")
+    tctx.autotune(1, 32, 256, path)
+    with open(path) as f:
+        cache = json.load(f)
+    assert cache["kind"] == "lu-ir" and len(cache["workgroups_per_cu"]) == 3
+    assert all(g >= 1 for g in cache["workgroups_per_cu"])
+    after = solve_batch(torch, tctx, bt, la_hip.PREC_FP32)[0]
+    for u, v in zip(before, after):
+        assert np.array_equal(u, v)
+    c2 = la_hip.Context(0)
+    try:
+        c2.autotune_load(path)
+        plan = c2.batch_plan(1, sizes, nrhs, la_hip.PREC_FP16)
+        assert "autotuned" in plan.describe()
+        plan.close()
+    finally:
+        c2.close()
