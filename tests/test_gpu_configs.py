@@ -4,7 +4,9 @@
   2024): the permutation vector must be identical to the oracle's (the
   C restatement of LUDecomposition::new, src/decomp/lu.rs:104-168) recorded
   in tests/golden/cfg2_n8192.json together with its tie-freeness
-  certificate; the solution is compared with the oracle's on every 32nd row.
+  certificate; both solutions must be backward stable, and the solution is
+  compared with the oracle's on every 32nd row (within 1e-10: the order of
+  rounding differs and kappa of the matrix amplifies it).
 * DGEMM 8192 x 8192 x 8192 (the Mul of src/matrix/mod.rs:957-980) on
   standard normal operands: 64 fixed rows of C against the oracle's
   sequential-sum product of those rows.
@@ -42,7 +44,8 @@ def test_cfg2_lu_pivots_and_solution(tctx, torch_cuda):
     print(f"cfg2 LU: n={n} seed={seed}", flush=True)
     a = oracle.normal(seed, fx["streams"]["A"], (n, n))
     b = oracle.normal(seed, fx["streams"]["B"], (n, 16))
-    A = torch.from_numpy(a).cuda()
+    A0 = torch.from_numpy(a).cuda()
+    A = A0.clone()
     B = torch.from_numpy(b).cuda()
     del a
     piv = torch.empty(n, dtype=torch.int64, device="cuda")
@@ -58,12 +61,21 @@ def test_cfg2_lu_pivots_and_solution(tctx, torch_cuda):
     ref = np.array(fx["piv"], dtype=np.uint64)
     first = np.nonzero(p != ref)[0]
     assert first.size == 0, f"pivot vector differs from the oracle first at row {first[:5]}"
+    # both solutions are backward stable (the oracle's eta is in the fixture)
+    eta = tctx.backward_error_dev(A0, X, B)
+    assert eta <= 1e-14 and max(fx["x_backward_error"]) <= 1e-14
+    # With identical pivots the two solutions differ only by the order of
+    # rounding (blocked MFMA sums vs the reference's sequential dots); that
+    # difference is amplified by the condition of A (kappa_2 of an n = 8192
+    # standard normal matrix is ~1e4 and heavy-tailed), so it sits near
+    # 10 kappa u = 2e-11 here, above the 1e-12 that holds at n = 256.
     rows = fx["x_rows"]
     xs = X[rows].cpu().numpy()
     xr = np.array(fx["x_sample"])
     err = np.abs(xs - xr).max(axis=0) / np.array(fx["x_col_inf"])
-    print("cfg2 solution rel. difference per column:", err.max(), flush=True)
-    assert err.max() <= 1e-12, err
+    print(f"cfg2 eta {eta:.2e}; solution rel. difference per column: max {err.max():.2e}",
+          flush=True)
+    assert err.max() <= 1e-10, err
 
 
 def test_cfg2_dgemm_8192(tctx, torch_cuda):
