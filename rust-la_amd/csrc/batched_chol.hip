@@ -22,9 +22,9 @@
 //      (row per thread), the panel and the zeros right of the diagonal block
 //      go to L, wave 0 solves this block of y, the rows below update their
 //      part of b from registers;
-//   backward substitution block by block: the rows below contribute through
-//   a coalesced column sweep of L, then one wave solves the transposed
-//   diagonal block with readlane broadcasts.
+//   backward substitution block by block, last block first: one wave solves
+//   the transposed diagonal block with readlane broadcasts, then the block's
+//   rows of L are pushed into the part of y above it in one coalesced sweep.
 // L is both output and left-looking operand; it is written and re-read by
 // the same workgroup with a workgroup barrier in between.
 #include "la_common.hpp"
@@ -44,7 +44,6 @@ __host__ __device__ inline int chol_bucket(int n) {
 struct CbLds {
     double P[CB_MAXN][CB_NBMAX + 1];   // current panel, rows J0..n-1
     double y[CB_MAXN];                 // b -> y -> x
-    double red[CB_NT];                 // backward partial sums, [row group][column]
     double dinv[CB_NBMAX];             // reciprocals of the diagonal block's pivots
     int bad, prob;
 };
@@ -59,7 +58,7 @@ __device__ inline double cb_bcast(double v, int lane) {
 // NB: panel width; RTW: row tiles whose MFMA K loops run together (more
 // loads in flight per round trip against more registers)
 template <int NB, int CB_RTW>
-__device__ void chol_solve_one(int n, const double *__restrict__ Ap, double *Lp,
+__device__ __forceinline__ void chol_solve_one(int n, const double *__restrict__ Ap, double *Lp,
                                const double *__restrict__ bp, double *xp, int32_t *status,
                                CbLds &S)
 {
@@ -78,7 +77,7 @@ __device__ void chol_solve_one(int n, const double *__restrict__ Ap, double *Lp,
         const int rt_n = (m + 15) >> 4;
         int asym = 0;
         const bool okB = l16 < jb;
-        const double *pb = Lp + (int64_t)(okB ? J0 + l16 : 0) * n + 4 * lhi;
+        const double *pb = Lp + (int64_t)(J0 + min(l16, jb - 1)) * n;
         for (int pass = 0; pass < 4 / CB_RTW; ++pass) {
             if (wave + 4 * CB_RTW * pass >= rt_n) break;
             d4_t acc[CB_RTW];
@@ -91,26 +90,39 @@ __device__ void chol_solve_one(int n, const double *__restrict__ Ap, double *Lp,
                 for (int i = 0; i < CB_RTW; ++i) {
                     const int rowA = J0 + 16 * (wave + 4 * (CB_RTW * pass + i)) + l16;
                     okA[i] = rowA < n;
-                    pa[i] = Lp + (int64_t)(okA[i] ? rowA : 0) * n + 4 * lhi;
+                    pa[i] = Lp + (int64_t)min(rowA, n - 1) * n;
                 }
                 // k = k0 + 4*(lane/16) + s for MFMA step s, the same for A and B;
                 // J0 is a multiple of 8, so a K chunk of 16 may end inside the
                 // current panel (or past J0): those columns are masked out.
-                // KU chunks are loaded together before their MFMAs.
+                // KU chunks are loaded together before their MFMAs: every load
+                // is unconditional from a clamped column (< J0) of a clamped
+                // row, and out-of-range values are zeroed afterwards.
                 constexpr int KU = 4 / CB_RTW;
                 for (int k0 = 0; k0 < J0; k0 += 16 * KU) {
                     double a[KU][CB_RTW][4], b[KU][4];
 #pragma unroll
                     for (int u = 0; u < KU; ++u) {
-                        const int kk = k0 + 16 * u;
-                        const bool kin = kk + 4 * lhi < J0;
+                        // J0 is a multiple of 4: kc..kc+3 are all below J0 or none is
+                        const int kc = min(k0 + 16 * u + 4 * lhi, J0 - 4);
 #pragma unroll
-                        for (int s = 0; s < 4; ++s) b[u][s] = (okB && kin) ? pb[kk + s] : 0.0;
+                        for (int s = 0; s < 4; ++s) b[u][s] = pb[kc + s];
+#pragma unroll
+                        for (int i = 0; i < CB_RTW; ++i)
+#pragma unroll
+                            for (int s = 0; s < 4; ++s) a[u][i][s] = pa[i][kc + s];
+                    }
+#pragma unroll
+                    for (int u = 0; u < KU; ++u) {
+                        const bool kin = k0 + 16 * u + 4 * lhi < J0;
+#pragma unroll
+                        for (int s = 0; s < 4; ++s)
+                            if (!(okB && kin)) b[u][s] = 0.0;
 #pragma unroll
                         for (int i = 0; i < CB_RTW; ++i)
 #pragma unroll
                             for (int s = 0; s < 4; ++s)
-                                a[u][i][s] = (okA[i] && kin) ? pa[i][kk + s] : 0.0;
+                                if (!(okA[i] && kin)) a[u][i][s] = 0.0;
                     }
 #pragma unroll
                     for (int u = 0; u < KU; ++u)
@@ -237,35 +249,28 @@ __device__ void chol_solve_one(int n, const double *__restrict__ Ap, double *Lp,
         __syncthreads();
     }
 
-    // ---- backward substitution L^T x = y, last block first
-    constexpr int NG = CB_NT / NB;                 // row groups of the column sweep
+    // ---- backward substitution L^T x = y, last block first, right-looking:
+    // once block B of x is known, its rows of L are pushed into y[0, J0) at
+    // once.  The loads of the push (rows J0..J0+jb of L, one column per
+    // thread) and those of the diagonal block (wave 0) do not depend on x_B
+    // and are issued together at the start of the block, so a block costs
+    // one memory round trip.
     const int nblk = (n + NB - 1) / NB;
     for (int B = nblk - 1; B >= 0; --B) {
         const int J0 = B * NB, jb = min(NB, n - J0);
-        {
-            const int c = tid % NB, grp = tid / NB;
-            double s = 0.0;
-            if (c < jb) {
-#pragma unroll 2
-                for (int rr = J0 + jb + grp; rr < n; rr += NG)
-                    s = fma(Lp[(int64_t)rr * n + J0 + c], S.y[rr], s);
-            }
-            S.red[grp * NB + c] = s;
-        }
-        __syncthreads();
+        double lp[NB];
+        const int c = tid;                 // J0 <= CB_MAXN - NB < CB_NT
+        const bool push = c < J0;
+#pragma unroll
+        for (int rr = 0; rr < NB; ++rr)
+            lp[rr] = Lp[(int64_t)(J0 + min(rr, jb - 1)) * n + min(c, J0 > 0 ? J0 - 1 : 0)];
         if (wave == 0) {
             double colv[NB];          // lane c: column c of the diagonal block
 #pragma unroll
             for (int rr = 0; rr < NB; ++rr)
-                colv[rr] = (lane < jb && rr < jb && rr >= lane)
-                               ? Lp[(int64_t)(J0 + rr) * n + J0 + lane] : 0.0;
-            const double dgi = lane < jb ? 1.0 / Lp[(int64_t)(J0 + lane) * n + J0 + lane] : 0.0;
-            double xv = 0.0;
-            if (lane < jb) {
-                double red = 0.0;
-                for (int g = 0; g < NG; ++g) red += S.red[g * NB + lane];
-                xv = S.y[J0 + lane] - red;
-            }
+                colv[rr] = Lp[(int64_t)(J0 + min(rr, jb - 1)) * n + J0 + min(lane, jb - 1)];
+            const double dgi = 1.0 / Lp[(int64_t)(J0 + min(lane, jb - 1)) * n + J0 + min(lane, jb - 1)];
+            double xv = lane < jb ? S.y[J0 + lane] : 0.0;
 #pragma unroll
             for (int rr = NB - 1; rr >= 0; --rr) {
                 if (rr < jb) {
@@ -275,6 +280,14 @@ __device__ void chol_solve_one(int n, const double *__restrict__ Ap, double *Lp,
                 }
             }
             if (lane < jb) S.y[J0 + lane] = xv;
+        }
+        __syncthreads();
+        if (push) {
+            double s = 0.0;
+#pragma unroll
+            for (int rr = 0; rr < NB; ++rr)
+                if (rr < jb) s = fma(lp[rr], S.y[J0 + rr], s);
+            S.y[c] -= s;
         }
         __syncthreads();
     }

@@ -239,14 +239,17 @@ __device__ inline void put_value(uint64_t *w, uint64_t tag32, float v) {
 }
 
 #ifdef LAHIP_PANEL_TIMING
-// Phase clock of workgroup 0 (wall-clock ticks summed over columns):
-// [0] gather slots, [1] winner fetch, [2] swap, [3] critical wave, [4] bulk
+// Phase clock of workgroup 0, thread 0 (wall-clock ticks summed over
+// columns, kept in registers and added once at the end of the kernel):
+// [0] header poll, [1] winner-row fetch, [2] wait for the bulk update,
+// [3] swap, [4] multipliers and column j+1, [5] next local candidate,
+// [6] candidate-row update, [7] publication
 __device__ unsigned long long g_panel_phase[8];
 #define PHASE_MARK(k)                                                          \
     do {                                                                       \
         if (w == 0 && tid == 0) {                                              \
             const uint64_t now_ = wall_clock64();                              \
-            atomicAdd(&g_panel_phase[k], (unsigned long long)(now_ - ph_t));   \
+            ph_acc[k] += now_ - ph_t;                                          \
             ph_t = now_;                                                       \
         }                                                                      \
     } while (0)
@@ -337,6 +340,7 @@ void k_getrf_panel(T *A, int64_t lda, int M, int NB, int R, int kglob, int64_t *
     int pend_j = -1;          // step whose bulk update is still pending
 #ifdef LAHIP_PANEL_TIMING
     uint64_t ph_t = wall_clock64();
+    uint64_t ph_acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 #endif
     for (int j = 0; j < NB; ++j) {
         const uint32_t gs = (uint32_t)(kglob + j);
@@ -390,6 +394,7 @@ void k_getrf_panel(T *A, int64_t lda, int M, int NB, int R, int kglob, int64_t *
             }
             wave_argmax(ck, ci);
             abort = __any(abort);
+            PHASE_MARK(0);
             int p = ci;
             if (!abort && (p < j || p >= M)) {
                 if (lane == 0) atomicCAS((int *)status, 0, (int)LA_ERR_TIMEOUT);
@@ -441,6 +446,7 @@ void k_getrf_panel(T *A, int64_t lda, int M, int NB, int R, int kglob, int64_t *
                 s_piv[j] = p;
                 if (w == 0) ipiv[kglob + j] = (int64_t)kglob + p;
             }
+            PHASE_MARK(1);
         } else if (pend_j >= 0) {
             // 1'. meanwhile waves 1-3 finish step pend_j's rank-1 update:
             //     columns pend_j+2 .. NB-1 of every row below pend_j except
@@ -475,7 +481,7 @@ void k_getrf_panel(T *A, int64_t lda, int M, int NB, int R, int kglob, int64_t *
             }
         }
         __syncthreads();
-        PHASE_MARK(0);
+        PHASE_MARK(2);
         if (s_abort) return;
         const int p = s_p;
         const bool own_j = (j >= rowbase && j < rowbase + nrows);
@@ -492,7 +498,7 @@ void k_getrf_panel(T *A, int64_t lda, int M, int NB, int R, int kglob, int64_t *
             }
         }
         __syncthreads();
-        PHASE_MARK(2);
+        PHASE_MARK(3);
 
         // 4. critical path (wave 0): multipliers, column j+1, next candidate.
         //    This panel never runs for matrices small enough to be factored
@@ -521,9 +527,11 @@ void k_getrf_panel(T *A, int64_t lda, int M, int NB, int R, int kglob, int64_t *
             // the next reads cross lanes: keep the compiler from moving them
             // above the writes (LDS itself executes one wave's ops in order)
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            PHASE_MARK(4);
             if (j1 < NB) {
                 uint64_t key; int idx; bool neg;
                 local_cand(j1, key, idx, neg);
+                PHASE_MARK(5);
                 // bring the candidate row and row j1 fully up to date
                 const int rc = (idx != IDX_NONE) ? idx - rowbase : -1;
                 const int rd = (j1 >= rowbase && j1 < rowbase + nrows && j1 != idx) ? j1 - rowbase : -1;
@@ -532,15 +540,20 @@ void k_getrf_panel(T *A, int64_t lda, int M, int NB, int R, int kglob, int64_t *
                     if (rd >= 0) blk[rd * LDB + c] -= blk[rd * LDB + j] * urow[c];
                 }
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                PHASE_MARK(6);
                 publish(j1, key, idx, neg);
                 if (lane == 0) { s_skip[0] = rc; s_skip[1] = rd; }
             }
+            PHASE_MARK(7);
         }
         __syncthreads();
-        PHASE_MARK(3);
         pend_j = (j1 + 1 < NB) ? j : -1;
     }
 
+#ifdef LAHIP_PANEL_TIMING
+    if (w == 0 && tid == 0)
+        for (int k = 0; k < 8; ++k) atomicAdd(&g_panel_phase[k], (unsigned long long)ph_acc[k]);
+#endif
     for (int e = tid; e < nrows * NB; e += blockDim.x) {
         int r = e / NB, c = e - r * NB;
         A[(int64_t)(rowbase + r) * lda + c] = blk[r * LDB + c];
