@@ -13,15 +13,17 @@
 //   1. P = A[J0:n, J0:J0+NB] - L[J0:n, 0:J0] L[J0:J0+NB, 0:J0]^T on
 //      v_mfma_f64_16x16x4 straight from the L written so far, A read in
 //      the accumulator layout, every strictly-lower element compared with
-//      its mirror; P goes to LDS;
-//   2. one wave factors the NB x NB diagonal block: lane r holds row r, the
+//      its mirror; P goes to LDS.  Wave 0 updates only the first row tile
+//      (the diagonal block's rows) and goes straight on to step 2 while
+//      waves 1-3 update the other row tiles, so the serial diagonal block
+//      is hidden behind the rest of the panel update;
+//   2. wave 0 factors the NB x NB diagonal block: lane r holds row r, the
 //      column being eliminated is always register 0 (rows rotate left one
 //      step per column) and column values are broadcast with readlane, so
-//      the block needs no barrier;
+//      the block needs no barrier; then it solves this block of y;
 //   3. every row below solves against the diagonal block in registers
 //      (row per thread), the panel and the zeros right of the diagonal block
-//      go to L, wave 0 solves this block of y, the rows below update their
-//      part of b from registers;
+//      go to L, the rows below update their part of b from registers;
 //   backward substitution block by block, last block first: one wave solves
 //   the transposed diagonal block with readlane broadcasts, then the block's
 //   rows of L are pushed into the part of y above it in one coalesced sweep.
@@ -29,6 +31,7 @@
 // the same workgroup with a workgroup barrier in between.
 #include "la_common.hpp"
 #include <cmath>
+#include <type_traits>
 
 namespace lahip {
 
@@ -70,101 +73,110 @@ __device__ __forceinline__ void chol_solve_one(int n, const double *__restrict__
 
     for (int J0 = 0; J0 < n; J0 += NB) {
         const int jb = min(NB, n - J0), m = n - J0;
-        // ---- 1. panel update on MFMA, symmetry check, P -> LDS
-        // wave w owns row tiles w, w+4, w+8, w+12 (m <= 256) and runs the
-        // K loops of CB_RTW of them together (4 + 4*CB_RTW independent loads
-        // per lane per 16-wide chunk)
+        // ---- 1. panel update on MFMA, symmetry check, P -> LDS, and (wave 0)
+        //         the diagonal block right after its own row tile
+        // Wave 0 updates row tile 0 (the diagonal block's rows) alone and goes
+        // on to factor the diagonal block while waves 1-3 update row tiles
+        // w, w+3, w+6, ... (m <= 256), running the K loops of RTW of them
+        // together (4 + 4*RTW independent loads per lane per 16-wide chunk).
         const int rt_n = (m + 15) >> 4;
         int asym = 0;
         const bool okB = l16 < jb;
         const double *pb = Lp + (int64_t)(J0 + min(l16, jb - 1)) * n;
-        for (int pass = 0; pass < 4 / CB_RTW; ++pass) {
-            if (wave + 4 * CB_RTW * pass >= rt_n) break;
-            d4_t acc[CB_RTW];
+        auto update_tiles = [&](auto rtw_c, int base, int step) {
+            constexpr int RTW = decltype(rtw_c)::value;
+            for (int pass = 0;; ++pass) {
+                if (base + step * RTW * pass >= rt_n) break;
+                d4_t acc[RTW];
 #pragma unroll
-            for (int i = 0; i < CB_RTW; ++i) acc[i] = (d4_t){0.0, 0.0, 0.0, 0.0};
-            if (J0 > 0) {
-                const double *pa[CB_RTW];
-                bool okA[CB_RTW];
+                for (int i = 0; i < RTW; ++i) acc[i] = (d4_t){0.0, 0.0, 0.0, 0.0};
+                if (J0 > 0) {
+                    const double *pa[RTW];
+                    bool okA[RTW];
 #pragma unroll
-                for (int i = 0; i < CB_RTW; ++i) {
-                    const int rowA = J0 + 16 * (wave + 4 * (CB_RTW * pass + i)) + l16;
-                    okA[i] = rowA < n;
-                    pa[i] = Lp + (int64_t)min(rowA, n - 1) * n;
-                }
-                // k = k0 + 4*(lane/16) + s for MFMA step s, the same for A and B;
-                // J0 is a multiple of 8, so a K chunk of 16 may end inside the
-                // current panel (or past J0): those columns are masked out.
-                // KU chunks are loaded together before their MFMAs: every load
-                // is unconditional from a clamped column (< J0) of a clamped
-                // row, and out-of-range values are zeroed afterwards.
-                constexpr int KU = 4 / CB_RTW;
-                for (int k0 = 0; k0 < J0; k0 += 16 * KU) {
-                    double a[KU][CB_RTW][4], b[KU][4];
-#pragma unroll
-                    for (int u = 0; u < KU; ++u) {
-                        // J0 is a multiple of 4: kc..kc+3 are all below J0 or none is
-                        const int kc = min(k0 + 16 * u + 4 * lhi, J0 - 4);
-#pragma unroll
-                        for (int s = 0; s < 4; ++s) b[u][s] = pb[kc + s];
-#pragma unroll
-                        for (int i = 0; i < CB_RTW; ++i)
-#pragma unroll
-                            for (int s = 0; s < 4; ++s) a[u][i][s] = pa[i][kc + s];
+                    for (int i = 0; i < RTW; ++i) {
+                        const int rowA = J0 + 16 * (base + step * (RTW * pass + i)) + l16;
+                        okA[i] = rowA < n;
+                        pa[i] = Lp + (int64_t)min(rowA, n - 1) * n;
                     }
+                    // k = k0 + 4*(lane/16) + s for MFMA step s, the same for A
+                    // and B; J0 is a multiple of 8, so a K chunk of 16 may end
+                    // inside the current panel (or past J0): those columns are
+                    // masked out.  KU chunks are loaded together before their
+                    // MFMAs: every load is unconditional from a clamped column
+                    // (< J0) of a clamped row, and out-of-range values are
+                    // zeroed afterwards.
+                    constexpr int KU = 4 / RTW;
+                    for (int k0 = 0; k0 < J0; k0 += 16 * KU) {
+                        double a[KU][RTW][4], b[KU][4];
 #pragma unroll
-                    for (int u = 0; u < KU; ++u) {
-                        const bool kin = k0 + 16 * u + 4 * lhi < J0;
+                        for (int u = 0; u < KU; ++u) {
+                            // J0 is a multiple of 4: kc..kc+3 all below J0 or none
+                            const int kc = min(k0 + 16 * u + 4 * lhi, J0 - 4);
 #pragma unroll
-                        for (int s = 0; s < 4; ++s)
-                            if (!(okB && kin)) b[u][s] = 0.0;
+                            for (int s = 0; s < 4; ++s) b[u][s] = pb[kc + s];
 #pragma unroll
-                        for (int i = 0; i < CB_RTW; ++i)
+                            for (int i = 0; i < RTW; ++i)
+#pragma unroll
+                                for (int s = 0; s < 4; ++s) a[u][i][s] = pa[i][kc + s];
+                        }
+#pragma unroll
+                        for (int u = 0; u < KU; ++u) {
+                            const bool kin = k0 + 16 * u + 4 * lhi < J0;
 #pragma unroll
                             for (int s = 0; s < 4; ++s)
-                                if (!(okA[i] && kin)) a[u][i][s] = 0.0;
+                                if (!(okB && kin)) b[u][s] = 0.0;
+#pragma unroll
+                            for (int i = 0; i < RTW; ++i)
+#pragma unroll
+                                for (int s = 0; s < 4; ++s)
+                                    if (!(okA[i] && kin)) a[u][i][s] = 0.0;
+                        }
+#pragma unroll
+                        for (int u = 0; u < KU; ++u)
+#pragma unroll
+                            for (int i = 0; i < RTW; ++i)
+#pragma unroll
+                                for (int s = 0; s < 4; ++s)
+                                    acc[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                                        a[u][i][s], b[u][s], acc[i], 0, 0, 0);
                     }
-#pragma unroll
-                    for (int u = 0; u < KU; ++u)
-#pragma unroll
-                        for (int i = 0; i < CB_RTW; ++i)
-#pragma unroll
-                            for (int s = 0; s < 4; ++s)
-                                acc[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u][i][s], b[u][s],
-                                                                              acc[i], 0, 0, 0);
                 }
-            }
 #pragma unroll
-            for (int i = 0; i < CB_RTW; ++i) {
-                const int rt = wave + 4 * (CB_RTW * pass + i);
-                if (rt < rt_n) {
-                    // the element and its mirror, loaded unconditionally from
-                    // clamped (in-bounds) positions so that the eight loads
-                    // issue together; out-of-range lanes discard them
-                    double v[4], w[4];
-                    const int gcc = J0 + min(l16, jb - 1);
+                for (int i = 0; i < RTW; ++i) {
+                    const int rt = base + step * (RTW * pass + i);
+                    if (rt < rt_n) {
+                        // the element and its mirror, loaded unconditionally
+                        // from clamped (in-bounds) positions so that the eight
+                        // loads issue together; out-of-range lanes discard them
+                        double v[4], w[4];
+                        const int gcc = J0 + min(l16, jb - 1);
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int grc = J0 + min(16 * rt + lhi + 4 * r, m - 1);
-                        v[r] = Ap[(int64_t)grc * n + gcc];
-                        w[r] = Ap[(int64_t)gcc * n + grc];
-                    }
+                        for (int r = 0; r < 4; ++r) {
+                            const int grc = J0 + min(16 * rt + lhi + 4 * r, m - 1);
+                            v[r] = Ap[(int64_t)grc * n + gcc];
+                            w[r] = Ap[(int64_t)gcc * n + grc];
+                        }
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int pr = 16 * rt + lhi + 4 * r;      // f64 MFMA row layout
-                        if (pr < m && l16 < jb) {
-                            const int gr = J0 + pr, gc = J0 + l16;
-                            if (gr > gc && !(v[r] == w[r])) asym = 1;   // NaN too
-                            S.P[pr][l16] = v[r] - acc[i][r];
+                        for (int r = 0; r < 4; ++r) {
+                            const int pr = 16 * rt + lhi + 4 * r;      // f64 MFMA row layout
+                            if (pr < m && l16 < jb) {
+                                const int gr = J0 + pr, gc = J0 + l16;
+                                if (gr > gc && !(v[r] == w[r])) asym = 1;   // NaN too
+                                S.P[pr][l16] = v[r] - acc[i][r];
+                            }
                         }
                     }
                 }
             }
-        }
-        if (asym) atomicOr(&S.bad, 1);
-        __syncthreads();
-        // ---- 2. diagonal block, one wave, rows in registers (rotated)
+        };
         if (wave == 0) {
+            update_tiles(std::integral_constant<int, 1>{}, 0, 1 << 20);   // tile 0 only
+            // the diagonal block's rows were written by this wave's own lanes:
+            // keep the compiler from moving the reads above the writes (LDS
+            // executes one wave's operations in order)
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            // ---- 2. diagonal block, rows in registers (rotated)
             double a[NB];
 #pragma unroll
             for (int c = 0; c < NB; ++c) a[c] = (lane < jb && c < jb) ? S.P[lane][c] : 0.0;
@@ -198,7 +210,22 @@ __device__ __forceinline__ void chol_solve_one(int n, const double *__restrict__
                 for (int c = 0; c < NB; ++c)
                     if (c <= lane) S.P[lane][c] = a[c];
             }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            // this block of y: forward substitution with the diagonal block
+            double yv = lane < jb ? S.y[J0 + lane] : 0.0;
+#pragma unroll
+            for (int c = 0; c < NB; ++c) {
+                if (c < jb) {
+                    if (lane == c) yv = yv * S.dinv[c];
+                    const double yc = cb_bcast(yv, c);
+                    if (lane > c && lane < jb) yv -= S.P[lane][c] * yc;
+                }
+            }
+            if (lane < jb) S.y[J0 + lane] = yv;
+        } else {
+            update_tiles(std::integral_constant<int, CB_RTW>{}, wave, 3);
         }
+        if (asym) atomicOr(&S.bad, 1);
         __syncthreads();
         // ---- 3. rows below: solve against the diagonal block; write L
         const int r = jb + tid;
@@ -224,19 +251,6 @@ __device__ __forceinline__ void chol_solve_one(int n, const double *__restrict__
         for (int e = tid; e < jb * m; e += CB_NT) {
             const int rr = e / m, c = e - rr * m;
             Lp[(int64_t)(J0 + rr) * n + J0 + c] = (c < jb && c <= rr) ? S.P[rr][c] : 0.0;
-        }
-        // this block of y (wave 0): forward substitution with the diagonal block
-        if (wave == 0) {
-            double yv = lane < jb ? S.y[J0 + lane] : 0.0;
-#pragma unroll
-            for (int c = 0; c < NB; ++c) {
-                if (c < jb) {
-                    if (lane == c) yv = yv * S.dinv[c];
-                    const double yc = cb_bcast(yv, c);
-                    if (lane > c && lane < jb) yv -= S.P[lane][c] * yc;
-                }
-            }
-            if (lane < jb) S.y[J0 + lane] = yv;
         }
         __syncthreads();
         if (r < m) {
