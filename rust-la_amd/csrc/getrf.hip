@@ -73,6 +73,27 @@ __device__ inline void wave_argmax(uint64_t &key, int &idx) {
     idx = (int)~nidx;
 }
 
+// Value of row r of a wave's rows (lane r % 64, register r / 64) broadcast
+// to every lane; r is the same in every lane.
+__device__ inline float bcast_lane(float x, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+}
+__device__ inline double bcast_lane(double x, int l) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane((int)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+template <typename T, int RPL>
+__device__ inline T lane_value(const T (&v)[RPL], int r) {
+    const int q = r >> 6;
+    T x = v[0];
+#pragma unroll
+    for (int i = 1; i < RPL; ++i)
+        if (q == i) x = v[i];
+    return bcast_lane(x, r & 63);
+}
+
 // Workgroup argmax; every thread returns the winner.  Uses two barriers.
 __device__ inline void wg_argmax(uint64_t &key, int &idx, uint64_t *s_key, int *s_idx) {
     wave_argmax(key, idx);
@@ -312,6 +333,25 @@ void k_getrf_panel(T *A, int64_t lda, int M, int NB, int R, int kglob, int64_t *
             st_dev(slot + 1, tag32 | (neg ? 0x80000000ull : 0ull) | (uint32_t)idx);
         }
     };
+    // publish the candidate of step s from registers: lane c holds column c
+    // of the candidate row (vc) and of row s (vd, used if this workgroup
+    // owns row s)
+    auto publish_row = [&](int s, uint64_t key, int idx, bool neg, T vc, T vd) {
+        const uint32_t gs = (uint32_t)(kglob + s);
+        const int buf = gs & 1;
+        const uint64_t tag32 = (uint64_t)gs << 32;
+        uint64_t *slot = cand + ((size_t)buf * PANEL_MAXP + w) * CAND_WORDS;
+        if (lane < NB) {
+            if (idx != IDX_NONE) put_value(slot + 2 + W * lane, tag32, vc);
+            if (s >= rowbase && s < rowbase + nrows)
+                put_value(diagw + (size_t)buf * W * NB_IN + W * lane, tag32, vd);
+        }
+        if (lane == 0) {
+            const uint64_t tagb = ((gs >> 1) & 1) ? TAG_BIT : 0;
+            st_dev(slot + 0, tagb | key);
+            st_dev(slot + 1, tag32 | (neg ? 0x80000000ull : 0ull) | (uint32_t)idx);
+        }
+    };
     // local candidate of column s over rows >= s (wave 0, RPL rows per lane)
     auto local_cand = [&](int s, uint64_t &key, int &idx, bool &neg) {
         key = 0; idx = IDX_NONE;
@@ -336,7 +376,6 @@ void k_getrf_panel(T *A, int64_t lda, int M, int NB, int R, int kglob, int64_t *
         local_cand(0, key, idx, neg);
         publish(0, key, idx, neg);
     }
-    __shared__ int s_p;
     int pend_j = -1;          // step whose bulk update is still pending
 #ifdef LAHIP_PANEL_TIMING
     uint64_t ph_t = wall_clock64();
@@ -442,11 +481,24 @@ void k_getrf_panel(T *A, int64_t lda, int M, int NB, int R, int kglob, int64_t *
             }
             if (lane == 0) {
                 s_abort = abort;
-                s_p = p;
                 s_piv[j] = p;
                 if (w == 0) ipiv[kglob + j] = (int64_t)kglob + p;
             }
             PHASE_MARK(1);
+            // 3. swap rows j and p inside the panel, here rather than after
+            //    the barrier: waves 1-3 never write those two rows in the
+            //    meantime (they are the rows the critical wave of step j - 1
+            //    brought up to date itself), so the swap costs no barrier
+            if (!abort && p != j) {
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                const bool own_j = (j >= rowbase && j < rowbase + nrows);
+                const bool own_p = (p >= rowbase && p < rowbase + nrows);
+                if (lane < NB) {
+                    if (own_j) blk[(j - rowbase) * LDB + lane] = urow[lane];
+                    if (own_p) blk[(p - rowbase) * LDB + lane] = drow[lane];
+                }
+            }
+            PHASE_MARK(3);
         } else if (pend_j >= 0) {
             // 1'. meanwhile waves 1-3 finish step pend_j's rank-1 update:
             //     columns pend_j+2 .. NB-1 of every row below pend_j except
@@ -483,31 +535,20 @@ void k_getrf_panel(T *A, int64_t lda, int M, int NB, int R, int kglob, int64_t *
         __syncthreads();
         PHASE_MARK(2);
         if (s_abort) return;
-        const int p = s_p;
-        const bool own_j = (j >= rowbase && j < rowbase + nrows);
-        const bool own_p = (p != j) && (p >= rowbase && p < rowbase + nrows);
-        // 3. swap rows j and p inside the panel
-        if (p != j) {
-            if (own_j) {
-                const int rl = j - rowbase;
-                for (int c = tid; c < NB; c += blockDim.x) blk[rl * LDB + c] = urow[c];
-            }
-            if (own_p) {
-                const int rl = p - rowbase;
-                for (int c = tid; c < NB; c += blockDim.x) blk[rl * LDB + c] = drow[c];
-            }
-        }
-        __syncthreads();
-        PHASE_MARK(3);
 
         // 4. critical path (wave 0): multipliers, column j+1, next candidate.
         //    This panel never runs for matrices small enough to be factored
         //    in one piece, so the multipliers use one reciprocal per column.
+        //    The new column j+1 stays in registers for the candidate search,
+        //    the candidate's sign and the multipliers of the two rows brought
+        //    up to date come from their lanes by readlane, and the updated
+        //    rows are published from registers: two LDS round trips in all.
         const T piv = urow[j];
         const int j1 = j + 1;
         if (wave == 0) {
             const T rpiv = (piv != T(0)) ? T(1) / piv : T(1);
             const T u1 = (j1 < NB) ? urow[j1] : T(0);
+            const T uc = urow[min(lane, NB - 1)];       // column lane of the winner row
             T lv[RPL], av[RPL];
 #pragma unroll
             for (int q = 0; q < RPL; ++q) {
@@ -520,28 +561,53 @@ void k_getrf_panel(T *A, int64_t lda, int M, int NB, int R, int kglob, int64_t *
                 const int r = lane + 64 * q;
                 if (r < nrows && rowbase + r > j) {
                     const T l = lv[q] * rpiv;
+                    lv[q] = l;
                     blk[r * LDB + j] = l;
-                    if (j1 < NB) blk[r * LDB + j1] = av[q] - l * u1;
+                    if (j1 < NB) {
+                        av[q] = av[q] - l * u1;
+                        blk[r * LDB + j1] = av[q];
+                    }
                 }
             }
-            // the next reads cross lanes: keep the compiler from moving them
-            // above the writes (LDS itself executes one wave's ops in order)
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             PHASE_MARK(4);
             if (j1 < NB) {
-                uint64_t key; int idx; bool neg;
-                local_cand(j1, key, idx, neg);
-                PHASE_MARK(5);
-                // bring the candidate row and row j1 fully up to date
+                // local candidate of column j1 (rows >= j1) from the registers
+                uint64_t key = 0;
+                int idx = IDX_NONE;
+#pragma unroll
+                for (int q = 0; q < RPL; ++q) {
+                    const int r = lane + 64 * q, gr = rowbase + r;
+                    if (r < nrows && gr >= j1) {
+                        uint64_t k = abs_key(av[q]);
+                        if (gr == j1 && av[q] != av[q]) k = KEY_MAX;
+                        if (better(k, gr, key, idx)) { key = k; idx = gr; }
+                    }
+                }
+                wave_argmax(key, idx);
                 const int rc = (idx != IDX_NONE) ? idx - rowbase : -1;
                 const int rd = (j1 >= rowbase && j1 < rowbase + nrows && j1 != idx) ? j1 - rowbase : -1;
-                for (int c = j1 + 1 + lane; c < NB; c += 64) {
-                    if (rc >= 0) blk[rc * LDB + c] -= blk[rc * LDB + j] * urow[c];
-                    if (rd >= 0) blk[rd * LDB + c] -= blk[rd * LDB + j] * urow[c];
+                bool neg = false;
+                T lrc = T(0), lrd = T(0);
+                if (rc >= 0) {
+                    neg = lane_value(av, rc) < T(0);
+                    lrc = lane_value(lv, rc);
                 }
+                if (rd >= 0) lrd = lane_value(lv, rd);
+                PHASE_MARK(5);
+                // bring the candidate row and row j1 fully up to date (their
+                // columns j and j1 were written above by their own lanes)
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                T vc = T(0), vd = T(0);
+                if (lane < NB) {
+                    if (rc >= 0) vc = blk[rc * LDB + lane];
+                    if (rd >= 0) vd = blk[rd * LDB + lane];
+                    if (lane > j1) {
+                        if (rc >= 0) { vc -= lrc * uc; blk[rc * LDB + lane] = vc; }
+                        if (rd >= 0) { vd -= lrd * uc; blk[rd * LDB + lane] = vd; }
+                    }
+                }
                 PHASE_MARK(6);
-                publish(j1, key, idx, neg);
+                publish_row(j1, key, idx, neg, vc, rd >= 0 ? vd : vc);
                 if (lane == 0) { s_skip[0] = rc; s_skip[1] = rd; }
             }
             PHASE_MARK(7);
