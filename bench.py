@@ -67,6 +67,44 @@ def cpu_baseline(n_sample):
             "seconds": dt}
 
 
+def cpu_sweep():
+    """The CPU oracle (one core, -O2, no FMA) over the sizes of the path:
+    LU + solve, Mul and Cholesky + solve at n = 256 .. 2048 (Mul to 1024),
+    and a 2000-problem sample of the batched Cholesky workload (sizes drawn
+    like the full 20000-problem batch).  About 10 s on the MI355X host."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle
+    out = {"unit": "GFLOP/s", "cores": 1, "kind": "port"}
+
+    def clock(fn):
+        t0 = time.perf_counter()
+        fn()
+        return time.perf_counter() - t0
+    lu, mm, ch = {}, {}, {}
+    for n in (256, 512, 1024, 2048):
+        a = oracle.uniform(7, 0, (n, n))
+        b = oracle.uniform(7, 2, (n, 1))
+        dt = clock(lambda: oracle.lu_solve(*oracle.lu_new(a)[:2], b))
+        lu[n] = {"s": dt, "gflops": lu_solve_flop(n) / dt / 1e9}
+        spd = oracle.spd_matrix(n, 9, 1)
+        dt = clock(lambda: oracle.chol_solve(oracle.chol_new(spd)[0], b))
+        ch[n] = {"s": dt, "gflops": (n ** 3 / 3 + 2 * n ** 2) / dt / 1e9}
+        if n <= 1024:
+            dt = clock(lambda: oracle.mul(a, a))
+            mm[n] = {"s": dt, "gflops": 2.0 * n ** 3 / dt / 1e9}
+    out["lu_solve"], out["cholesky_solve"], out["mul"] = lu, ch, mm
+    rng = np.random.default_rng(20000)
+    sizes = rng.integers(8, 257, size=2000)
+    mats = [oracle.spd_matrix(int(n), 4, i) for i, n in enumerate(sizes)]
+    rhs = [oracle.normal(4, 7000 + i, (int(n), 1)) for i, n in enumerate(sizes)]
+    dt = clock(lambda: [oracle.chol_solve(oracle.chol_new(m)[0], r) for m, r in zip(mats, rhs)])
+    out["batched_cholesky_sample"] = {
+        "problems": 2000, "n_range": [8, 256], "s": dt, "solves_per_s": 2000 / dt,
+        "note": "sizes uniform in [8, 256] as in the 20000-problem workload; one core"}
+    return out
+
+
 def timed(torch, fn, reps=1):
     """Mean GPU time (ms) of fn() over reps calls, HIP events on the current stream."""
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -238,7 +276,7 @@ def main():
     ap.add_argument("--n", type=int, default=16384)
     ap.add_argument("--kappa", type=float, default=1e6)
     ap.add_argument("--method", default="pivoted", choices=["pivoted", "tournament", "rbt"])
-    ap.add_argument("--cpu-n", type=int, default=2304)
+    ap.add_argument("--cpu-n", type=int, default=2560)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
@@ -432,6 +470,7 @@ def main():
             out["batched_lu_ir"] = batched_lu_ir(ctx, args.steps, args.warmup)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_n)
+        out["cpu_sweep"] = cpu_sweep()
         out["cpu_baseline"]["host_nproc"] = os.cpu_count()
         try:
             model = [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo")

@@ -164,7 +164,12 @@ int la_dgemm_dev(la_ctx *ctx, int64_t m, int64_t n, int64_t k,
 
 /* In-place LU of the m x n matrix at A (ld lda).  d_piv (device, length m)
  * receives the permutation vector; d_info (device int[2]) receives
- * {pospivsign, status}; status is LA_OK or LA_ERR_TIMEOUT. */
+ * {pospivsign, status}; status is LA_OK or LA_ERR_TIMEOUT.  Size limit (no
+ * counterpart in the reference, which has none): the column panels are
+ * factored by at most min(256, CUs) co-resident workgroups of at most 192
+ * rows each (FP64; 256 for the FP32 factor of la_dgesv_ir_dev), so
+ * m <= 49152 on a 256-CU MI355X (65536 for the FP32 factor); beyond it the
+ * call returns LA_ERR_ARG before launching anything. */
 int la_dgetrf_dev(la_ctx *ctx, int64_t m, int64_t n, double *A, int64_t lda,
                   uint64_t *d_piv, int32_t *d_info);
 
