@@ -224,6 +224,25 @@ def batched_lu_ir(ctx, steps, warmup, count=4096, seed=4096):
     return res
 
 
+def qr_bench(ctx, torch):
+    """Householder QR (QRDecomposition::new) of a standard-normal n x n
+    matrix on the device, n = 4096 and 8192: ms and TFLOP/s at (4/3) n^3."""
+    out = {}
+    for n in (4096, 8192):
+        A = torch.empty((n, n), dtype=torch.float64, device="cuda")
+        ctx.fill_normal_dev(31, 0, A)
+        W = torch.empty_like(A)
+        rd = torch.empty(n, dtype=torch.float64, device="cuda")
+        W.copy_(A)
+        ctx.geqrf_dev(W, rd)
+        W.copy_(A)
+        ms, _ = timed(torch, lambda: ctx.geqrf_dev(W, rd))
+        out[f"n{n}"] = {"ms": ms, "tflops": (4.0 / 3.0) * n ** 3 / (ms * 1e-3) / 1e12,
+                        "frac_of_fp64_peak": (4.0 / 3.0) * n ** 3 / (ms * 1e-3) / 1e12 / PEAKS["fp64"]}
+        del A, W
+    return out
+
+
 def cfg2(ctx, torch, reps):
     """n = 8192 plain FP64 LU + 16-RHS solve (standard normal A, B) and
     DGEMM 8192^3 on standard normal operands."""
@@ -465,6 +484,7 @@ def main():
             del A
         if not args.no_cfg2:
             out["cfg2_n8192"] = cfg2(ctx, torch, 2)
+            out["qr"] = qr_bench(ctx, torch)
         if not args.no_batched:
             out["batched_cholesky"] = batched_cholesky(ctx, args.steps, args.warmup)
             out["batched_lu_ir"] = batched_lu_ir(ctx, args.steps, args.warmup)

@@ -140,6 +140,40 @@ int la_dgesv_rm(la_ctx *ctx, int64_t n_rows, int64_t n_cols, const double *A,
  * (64 x 64 diagonal blocks inverted, then one GEMM per block). */
 int la_dinverse_rm(la_ctx *ctx, int64_t rows, int64_t cols, const double *A, double *Ainv);
 
+/* Householder QR.  Replaces `QRDecomposition::new` (src/decomp/qr.rs:26-107):
+ * QR (m x n) receives the reference's packed `qr` (R strictly above the
+ * diagonal, Householder vector u_k = x - a_k e_k on and below the diagonal
+ * of column k, columns with a_k = 0 left as they were) and rdiag
+ * (min(m, n)) its `rdiag` (a_k = -sign(x_kk)|x|, +|x| when x_kk <= 0).
+ * Blocked: 32-column panels, the panel's reflections applied to the rest as
+ * I - V T V^T on MFMA.  get_h, get_r and is_full_rank (qr.rs:109-147) are
+ * host post-processing of these two outputs; LA_ERR_SHAPE for an empty
+ * matrix (mod.rs:209). */
+int la_dgeqrf_rm(la_ctx *ctx, int64_t m, int64_t n, const double *A, double *QR,
+                 double *rdiag);
+/* `QRDecomposition::get_q` (qr.rs:149-186): Q (m x m) = the reflections
+ * applied, last first, to the identity restricted to its first min(m, n)
+ * diagonal entries. */
+int la_dqr_q_rm(la_ctx *ctx, int64_t m, int64_t n, const double *QR, const double *rdiag,
+                double *Q);
+/* `QRDecomposition::solve` (qr.rs:191-232): X (n x nx) minimising
+ * ||Q R X - B||_2.  LA_ERR_SHAPE if b_rows != m (qr.rs:193) or n > m
+ * (is_full_rank reads rdiag past its end there, a panic in the reference);
+ * LA_SINGULAR (-> None) if some rdiag[j] is exactly zero (qr.rs:194-196).
+ * Two deliberate differences, both defects of the reference: its "Y = Q^T B"
+ * step applies x - (<u,x>/u_kk) u, JAMA's formula for vectors normalised to
+ * |u|^2 = 2 u_kk, to the unnormalised vectors of qr.rs:75 (|u|^2 = -2 a u_kk),
+ * which is not the reflection; here Q^T B applies the reflections of
+ * qr.rs:91-104.  And for m > n the reference builds an n x nx matrix from
+ * m x nx values (a panic); here X is the n x nx least-squares solution. */
+int la_dqrs_rm(la_ctx *ctx, int64_t m, int64_t n, const double *QR, const double *rdiag,
+               int64_t b_rows, int64_t nx, const double *B, double *X);
+/* `Matrix::pinverse` (src/matrix/mod.rs:1049-1057): P (n x m) =
+ * (R^T R)^-1 A^T with R = get_r() of A's QR, the inverse by LU with the
+ * identity (mod.rs:1034-1037).  LA_SINGULAR where the reference's
+ * inverse().unwrap() panics (an exact zero pivot of R^T R). */
+int la_dpinverse_rm(la_ctx *ctx, int64_t m, int64_t n, const double *A, double *P);
+
 /* Cholesky factor L (full n x n, strict upper triangle zeroed).
  * Replaces `CholeskyDecomposition::new` (src/decomp/cholesky.rs:56-110):
  * LA_NOT_SQUARE (cholesky.rs:57-59), LA_NOT_SYMMETRIC (exact A[k][j] !=
@@ -193,6 +227,17 @@ int la_dgetrf_noalloc_dev(la_ctx *ctx, int64_t m, int64_t n, double *A, int64_t 
 int la_dgetrs_dev(la_ctx *ctx, int64_t m, int64_t n, const double *LU, int64_t ldlu,
                   const uint64_t *d_piv, int64_t nx, const double *B, int64_t ldb,
                   double *X, int64_t ldx, int32_t *d_status);
+
+/* Device-resident QR (la_dgeqrf_rm's factor in place of A, ld lda; d_rdiag
+ * device, min(m, n)), Q (m x m, ld ldq), and least-squares solve (X n x nx;
+ * B is not modified; status as la_dqrs_rm, checked before anything is
+ * launched). */
+int la_dgeqrf_dev(la_ctx *ctx, int64_t m, int64_t n, double *A, int64_t lda, double *d_rdiag);
+int la_dqr_q_dev(la_ctx *ctx, int64_t m, int64_t n, const double *QR, int64_t ldqr,
+                 const double *d_rdiag, double *Q, int64_t ldq);
+int la_dqrs_dev(la_ctx *ctx, int64_t m, int64_t n, const double *QR, int64_t ldqr,
+                const double *d_rdiag, int64_t nx, const double *B, int64_t ldb, double *X,
+                int64_t ldx);
 
 /* Device-resident inverse: A (ld lda) is overwritten with its packed LU,
  * d_piv receives the permutation, X (ld must equal n) the inverse; d_info

@@ -8,6 +8,8 @@
 //                                 src/matrix/mod.rs:1025-1047
 //   la::LUDecomposition<T>        src/decomp/lu.rs:95-279
 //   la::CholeskyDecomposition<T>  src/decomp/cholesky.rs:51-145
+//   la::QRDecomposition<T>        src/decomp/qr.rs:19-233
+//   Matrix::pinverse              src/matrix/mod.rs:1049-1057
 // Reference panics (assert!) throw la::Panic; Option::None is an empty
 // std::optional.  Every f64 factorisation, solve and product runs on the
 // GPU through the C ABI; when no device or library is present the call
@@ -116,6 +118,8 @@ public:
     std::optional<Matrix> inverse() const;
     bool is_singular() const { return !is_non_singular(); }
     bool is_non_singular() const;
+    // pinverse (mod.rs:1049-1057), f64
+    Matrix pinverse() const;
 
 private:
     size_t no_rows_;
@@ -274,6 +278,79 @@ private:
     Matrix<T> l_;
 };
 
+// ---- QRDecomposition (qr.rs:19-233) ---------------------------------------
+template <typename T>
+class QRDecomposition {
+    static_assert(std::is_same<T, double>::value, "the GPU QR path is f64");
+
+public:
+    // QRDecomposition::new (qr.rs:26-44)
+    explicit QRDecomposition(const Matrix<T> &a) : qr_(a) {
+        const size_t m = a.rows(), n = a.cols();
+        std::vector<T> qr(m * n);
+        rdiag_.resize(std::min(m, n));
+        check(la_dgeqrf_rm(thread_ctx(), (int64_t)m, (int64_t)n, a.get_data().data(), qr.data(),
+                           rdiag_.data()),
+              "QRDecomposition::new");
+        qr_ = Matrix<T>(m, n, std::move(qr));
+    }
+    // is_full_rank (qr.rs:109-116): rdiag is indexed for every column
+    bool is_full_rank() const {
+        for (size_t j = 0; j < qr_.cols(); ++j) {
+            if (j >= rdiag_.size()) throw Panic("is_full_rank: index out of bounds");
+            if (rdiag_[j] == T(0)) return false;
+        }
+        return true;
+    }
+    // get_h (qr.rs:118-132): the lower trapezoid of qr
+    Matrix<T> get_h() const {
+        const size_t m = qr_.rows(), n = qr_.cols();
+        std::vector<T> h(m * n);
+        for (size_t i = 0; i < m; ++i)
+            for (size_t j = 0; j < n; ++j) h[i * n + j] = i >= j ? qr_.get_data()[i * n + j] : T(0);
+        return Matrix<T>(m, n, std::move(h));
+    }
+    // get_r (qr.rs:134-147)
+    Matrix<T> get_r() const {
+        const size_t m = qr_.rows(), n = qr_.cols();
+        std::vector<T> r(m * n);
+        for (size_t i = 0; i < m; ++i)
+            for (size_t j = 0; j < n; ++j)
+                r[i * n + j] = i < j ? qr_.get_data()[i * n + j] : (i == j ? rdiag_[i] : T(0));
+        return Matrix<T>(m, n, std::move(r));
+    }
+    // get_q (qr.rs:149-186), m x m
+    Matrix<T> get_q() const {
+        const size_t m = qr_.rows(), n = qr_.cols();
+        std::vector<T> q(m * m);
+        check(la_dqr_q_rm(thread_ctx(), (int64_t)m, (int64_t)n, qr_.get_data().data(),
+                          rdiag_.data(), q.data()),
+              "QRDecomposition::get_q");
+        return Matrix<T>(m, m, std::move(q));
+    }
+    // solve (qr.rs:191-232) as a least-squares solve: n x nx, None when not
+    // of full rank; the reference's panics (b.rows() != rows, cols > rows)
+    // throw Panic.  Where the reference's formula is not the reflection, the
+    // reflections are applied (la_hip.h, la_dqrs_rm).
+    std::optional<Matrix<T>> solve(const Matrix<T> &b) const {
+        if (b.rows() != qr_.rows()) throw Panic("QRDecomposition::solve: b.rows() != rows");
+        if (!is_full_rank()) return std::nullopt;
+        const size_t m = qr_.rows(), n = qr_.cols(), nx = b.cols();
+        std::vector<T> x(n * nx);
+        const int st = la_dqrs_rm(thread_ctx(), (int64_t)m, (int64_t)n, qr_.get_data().data(),
+                                  rdiag_.data(), (int64_t)b.rows(), (int64_t)nx,
+                                  b.get_data().data(), x.data());
+        if (st == LA_SINGULAR) return std::nullopt;
+        check(st, "QRDecomposition::solve");
+        return Matrix<T>(n, nx, std::move(x));
+    }
+    const std::vector<T> &rdiag() const { return rdiag_; }
+
+private:
+    Matrix<T> qr_;
+    std::vector<T> rdiag_;
+};
+
 // ---- Matrix wrappers (mod.rs:1025-1047) ---------------------------------
 template <typename T>
 T Matrix<T>::det() const {
@@ -298,6 +375,16 @@ std::optional<Matrix<T>> Matrix<T>::inverse() const {
     } else {
         return LUDecomposition<T>(*this).solve(Matrix::id(no_rows_, no_rows_));
     }
+}
+template <typename T>
+Matrix<T> Matrix<T>::pinverse() const {
+    static_assert(std::is_same<T, double>::value, "the GPU pinverse path is f64");
+    const size_t m = no_rows_, n = cols();
+    std::vector<T> p(n * m);
+    const int st = la_dpinverse_rm(thread_ctx(), (int64_t)m, (int64_t)n, data_.data(), p.data());
+    if (st == LA_SINGULAR) throw Panic("pinverse: inverse().unwrap() on None");
+    check(st, "pinverse");
+    return Matrix(n, m, std::move(p));
 }
 template <typename T>
 bool Matrix<T>::is_non_singular() const {

@@ -21,6 +21,12 @@
  *   orc_smul           src/matrix/simd.rs:189-219    (Matrix<f32>::simd_mul)
  *   orc_sdot           src/matrix/simd.rs:166-187    (simd_dot, tail defect
  *                      corrected: the tail adds a[i]*b[i], see below)
+ *   orc_qr_new         src/decomp/qr.rs:26-107       (QRDecomposition::new)
+ *   orc_qr_is_full_rank qr.rs:109-116
+ *   orc_qr_q           src/decomp/qr.rs:149-186      (get_q)
+ *   orc_qr_solve_literal qr.rs:191-232 as written; orc_qr_solve the
+ *                      least-squares solve it means (see below)
+ *   orc_pinverse       src/matrix/mod.rs:1049-1057
  *
  * Status codes (mirroring the reference's failure signals):
  *   0  OK
@@ -420,4 +426,209 @@ float orc_sdot(int64_t len, const float *a, const float *b)
         sum += prod;
     }
     return sum;
+}
+
+/* ---- QRDecomposition::new (qr.rs:26-107): unblocked Householder, column by
+ * column.  qr (m*n) receives the packed result: R strictly above the
+ * diagonal, the Householder vector u_k = x - a e_k on and below the diagonal
+ * of column k (zero columns are left as they were); rdiag (min(m,n))
+ * receives a_k = -sign(x_kk) |x| (+|x| when x_kk <= 0). */
+int orc_qr_new(int64_t m, int64_t n, const double *a, double *qr, double *rdiag)
+{
+    if (m <= 0 || n <= 0) return ORC_SHAPE;        /* Matrix::new asserts */
+    memcpy(qr, a, (size_t)(m * n) * sizeof(double));
+    const int64_t diag_count = m < n ? m : n;
+    for (int64_t minor = 0; minor < diag_count; ++minor) {
+        double x_norm_sqr = 0.0;                   /* qr.rs:50-54 */
+        for (int64_t i = minor; i < m; ++i) {
+            double c = qr[i * n + minor];
+            x_norm_sqr = x_norm_sqr + c * c;
+        }
+        double av = qr[minor * n + minor] > 0.0 ? -sqrt(x_norm_sqr) : sqrt(x_norm_sqr);
+        rdiag[minor] = av;                         /* qr.rs:59-60 */
+        if (av != 0.0) {
+            qr[minor * n + minor] = qr[minor * n + minor] - av;   /* qr.rs:75 */
+            for (int64_t column = minor + 1; column < n; ++column) {   /* qr.rs:91-104 */
+                double x_dot_u = 0.0;
+                for (int64_t row = minor; row < m; ++row) {
+                    double p = qr[row * n + minor] * qr[row * n + column];
+                    x_dot_u = x_dot_u + p;
+                }
+                double den = av * qr[minor * n + minor];
+                double factor = x_dot_u / den;
+                for (int64_t row = minor; row < m; ++row) {
+                    double p = factor * qr[row * n + minor];
+                    qr[row * n + column] = qr[row * n + column] + p;
+                }
+            }
+        }
+    }
+    return ORC_OK;
+}
+
+/* ---- QRDecomposition::is_full_rank (qr.rs:109-116): 1 / 0; it indexes
+ * rdiag[j] for every column j, which is out of bounds (a panic) when n > m. */
+int orc_qr_is_full_rank(int64_t m, int64_t n, const double *rdiag)
+{
+    for (int64_t j = 0; j < n; ++j) {
+        if (j >= (m < n ? m : n)) return -1;
+        if (rdiag[j] == 0.0) return 0;
+    }
+    return 1;
+}
+
+/* ---- QRDecomposition::get_q (qr.rs:149-186): q (m*m) = H_0 H_1 ... applied
+ * to the identity restricted to the first min(m,n) diagonal entries. */
+int orc_qr_q(int64_t m, int64_t n, const double *qr, const double *rdiag, double *q)
+{
+    memset(q, 0, (size_t)(m * m) * sizeof(double));
+    const int64_t diag_count = m < n ? m : n;
+    for (int64_t minor = 0; minor < diag_count; ++minor) q[minor * m + minor] = 1.0;
+    for (int64_t minor = diag_count - 1; minor >= 0; --minor) {
+        if (qr[minor * n + minor] != 0.0) {
+            for (int64_t column = minor; column < m; ++column) {
+                double x_dot_u = 0.0;
+                for (int64_t row = minor; row < m; ++row) {
+                    double p = qr[row * n + minor] * q[row * m + column];
+                    x_dot_u = x_dot_u + p;
+                }
+                double den = rdiag[minor] * qr[minor * n + minor];
+                double factor = x_dot_u / den;
+                for (int64_t row = minor; row < m; ++row) {
+                    double p = factor * qr[row * n + minor];
+                    q[row * m + column] = q[row * m + column] + p;
+                }
+            }
+        }
+    }
+    return ORC_OK;
+}
+
+/* Shared checks of QRDecomposition::solve (qr.rs:191-232): b.rows() == m
+ * (assert :193), is_full_rank (:194-196, out-of-bounds panic when n > m). */
+static int orc_qr_solve_checks(int64_t m, int64_t n, const double *rdiag, int64_t b_rows)
+{
+    if (b_rows != m) return ORC_SHAPE;
+    int fr = orc_qr_is_full_rank(m, n, rdiag);
+    if (fr < 0) return ORC_SHAPE;
+    if (fr == 0) return ORC_SINGULAR;
+    return ORC_OK;
+}
+
+/* ---- QRDecomposition::solve exactly as written (qr.rs:191-232), for m == n
+ * (for m > n the reference's Matrix::new(n, nx, data of m*nx) asserts).  Its
+ * "Y = Q^T B" step applies x - (<u,x>/u_kk) u, JAMA's formula for vectors
+ * normalised to |u|^2 = 2 u_kk, to the vectors of qr.rs:75, whose norm is
+ * |u|^2 = -2 a u_kk: it is not the reflection unless a = -1.  Kept to show
+ * that defect; orc_qr_solve is the least-squares solve the routine means. */
+int orc_qr_solve_literal(int64_t m, int64_t n, const double *qr, const double *rdiag,
+                         int64_t b_rows, int64_t nx, const double *b, double *x)
+{
+    int st = orc_qr_solve_checks(m, n, rdiag, b_rows);
+    if (st) return st;
+    if (m != n) return ORC_SHAPE;
+    memcpy(x, b, (size_t)(m * nx) * sizeof(double));
+    for (int64_t k = 0; k < n; ++k)
+        for (int64_t j = 0; j < nx; ++j) {
+            double s = 0.0;
+            for (int64_t i = k; i < m; ++i) {
+                double p = qr[i * n + k] * x[i * nx + j];
+                s = s + p;
+            }
+            s = -s / qr[k * n + k];
+            for (int64_t i = k; i < m; ++i) {
+                double p = s * qr[i * n + k];
+                x[i * nx + j] = x[i * nx + j] + p;
+            }
+        }
+    for (int64_t k = n - 1; k >= 0; --k) {
+        for (int64_t j = 0; j < nx; ++j) x[k * nx + j] = x[k * nx + j] / rdiag[k];
+        for (int64_t i = 0; i < k; ++i)
+            for (int64_t j = 0; j < nx; ++j) {
+                double p = x[k * nx + j] * qr[i * n + k];
+                x[i * nx + j] = x[i * nx + j] - p;
+            }
+    }
+    return ORC_OK;
+}
+
+/* ---- The least-squares solve of QRDecomposition::solve with the
+ * reflections of qr.rs:75-104 (Hx = x + <x,u>/(a u_kk) u, as in new() and
+ * get_q), then R x = (Q^T b)[0..n) with R's diagonal rdiag.  m >= n; x is
+ * n*nx. */
+int orc_qr_solve(int64_t m, int64_t n, const double *qr, const double *rdiag,
+                 int64_t b_rows, int64_t nx, const double *b, double *x)
+{
+    int st = orc_qr_solve_checks(m, n, rdiag, b_rows);
+    if (st) return st;
+    double *y = (double *)malloc((size_t)(m * nx) * sizeof(double));
+    if (!y) return ORC_SHAPE;
+    memcpy(y, b, (size_t)(m * nx) * sizeof(double));
+    for (int64_t k = 0; k < n; ++k) {
+        if (qr[k * n + k] == 0.0) continue;
+        for (int64_t j = 0; j < nx; ++j) {
+            double s = 0.0;
+            for (int64_t i = k; i < m; ++i) {
+                double p = qr[i * n + k] * y[i * nx + j];
+                s = s + p;
+            }
+            double den = rdiag[k] * qr[k * n + k];
+            double f = s / den;
+            for (int64_t i = k; i < m; ++i) {
+                double p = f * qr[i * n + k];
+                y[i * nx + j] = y[i * nx + j] + p;
+            }
+        }
+    }
+    for (int64_t k = n - 1; k >= 0; --k) {
+        for (int64_t j = 0; j < nx; ++j) y[k * nx + j] = y[k * nx + j] / rdiag[k];
+        for (int64_t i = 0; i < k; ++i)
+            for (int64_t j = 0; j < nx; ++j) {
+                double p = y[k * nx + j] * qr[i * n + k];
+                y[i * nx + j] = y[i * nx + j] - p;
+            }
+    }
+    memcpy(x, y, (size_t)(n * nx) * sizeof(double));
+    free(y);
+    return ORC_OK;
+}
+
+/* ---- Matrix::pinverse (mod.rs:1049-1057): (R^T R)^-1 A^T with R = get_r()
+ * (qr.rs:134-147: m*n, rdiag on the diagonal, zero below), the products by
+ * impl Mul and the inverse by LU solve with the identity (mod.rs:1034-1037;
+ * .unwrap() on None is a panic: ORC_SINGULAR here).  p is n*m. */
+int orc_pinverse(int64_t m, int64_t n, const double *a, double *p)
+{
+    int st = ORC_OK;
+    double *qr = (double *)malloc((size_t)(m * n) * sizeof(double));
+    double *rd = (double *)malloc((size_t)((m < n ? m : n)) * sizeof(double));
+    double *r = (double *)malloc((size_t)(m * n) * sizeof(double));
+    double *rt = (double *)malloc((size_t)(m * n) * sizeof(double));
+    double *g = (double *)malloc((size_t)(n * n) * sizeof(double));
+    double *lu = (double *)malloc((size_t)(n * n) * sizeof(double));
+    uint64_t *piv = (uint64_t *)malloc((size_t)n * sizeof(uint64_t));
+    double *id = (double *)calloc((size_t)(n * n), sizeof(double));
+    double *gi = (double *)malloc((size_t)(n * n) * sizeof(double));
+    double *at = (double *)malloc((size_t)(m * n) * sizeof(double));
+    if (!qr || !rd || !r || !rt || !g || !lu || !piv || !id || !gi || !at) { st = ORC_SHAPE; goto done; }
+    if ((st = orc_qr_new(m, n, a, qr, rd))) goto done;
+    for (int64_t i = 0; i < m; ++i)
+        for (int64_t j = 0; j < n; ++j)
+            r[i * n + j] = i < j ? qr[i * n + j] : (i == j ? rd[i] : 0.0);
+    for (int64_t i = 0; i < m; ++i)
+        for (int64_t j = 0; j < n; ++j) rt[j * m + i] = r[i * n + j];
+    if ((st = orc_mul(n, m, m, n, rt, r, g))) goto done;
+    {
+        int sign;
+        double gap;
+        if ((st = orc_lu_new(n, n, g, lu, piv, &sign, &gap))) goto done;
+    }
+    for (int64_t i = 0; i < n; ++i) id[i * n + i] = 1.0;
+    if ((st = orc_lu_solve(n, n, lu, piv, n, n, id, gi))) goto done;
+    for (int64_t i = 0; i < m; ++i)
+        for (int64_t j = 0; j < n; ++j) at[j * m + i] = a[i * n + j];
+    st = orc_mul(n, n, n, m, gi, at, p);
+done:
+    free(qr); free(rd); free(r); free(rt); free(g); free(lu); free(piv); free(id); free(gi); free(at);
+    return st;
 }

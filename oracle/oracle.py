@@ -48,6 +48,12 @@ def lib():
         L.orc_smul.argtypes = [_i64, _i64, _i64, _i64, _fp, _fp, _fp]
         L.orc_sdot.argtypes = [_i64, _fp, _fp]
         L.orc_sdot.restype = ctypes.c_float
+        L.orc_qr_new.argtypes = [_i64, _i64, _dp, _dp, _dp]
+        L.orc_qr_is_full_rank.argtypes = [_i64, _i64, _dp]
+        L.orc_qr_q.argtypes = [_i64, _i64, _dp, _dp, _dp]
+        L.orc_qr_solve.argtypes = [_i64, _i64, _dp, _dp, _i64, _i64, _dp, _dp]
+        L.orc_qr_solve_literal.argtypes = L.orc_qr_solve.argtypes
+        L.orc_pinverse.argtypes = [_i64, _i64, _dp, _dp]
         _lib = L
     return _lib
 
@@ -242,3 +248,85 @@ def spd_matrix(n, seed, stream):
     """A = G G^T / n + I, G standard normal (stream)."""
     g = normal(seed, stream, (n, n))
     return g @ g.T / n + np.eye(n)
+
+
+def qr_new(a):
+    """QRDecomposition::new (src/decomp/qr.rs:26-107) -> (qr, rdiag)."""
+    a = _c(a)
+    m, n = a.shape
+    qr = np.empty_like(a)
+    rdiag = np.empty(min(m, n))
+    st = lib().orc_qr_new(m, n, _d(a), _d(qr), _d(rdiag))
+    if st:
+        raise OracleError(st)
+    return qr, rdiag
+
+
+def qr_is_full_rank(qr, rdiag):
+    """QRDecomposition::is_full_rank (qr.rs:109-116); raises on the reference's
+    out-of-bounds panic (n > m)."""
+    m, n = qr.shape
+    r = lib().orc_qr_is_full_rank(m, n, _d(_c(rdiag)))
+    if r < 0:
+        raise OracleError(SHAPE)
+    return bool(r)
+
+
+def qr_h(qr):
+    """QRDecomposition::get_h (qr.rs:118-132): the lower trapezoid of qr."""
+    return np.tril(qr)
+
+
+def qr_r(qr, rdiag):
+    """QRDecomposition::get_r (qr.rs:134-147): m x n, rdiag on the diagonal."""
+    r = np.triu(qr, 1)
+    k = len(rdiag)
+    r[np.arange(k), np.arange(k)] = rdiag
+    return r
+
+
+def qr_q(qr, rdiag):
+    """QRDecomposition::get_q (qr.rs:149-186): m x m."""
+    qr = _c(qr)
+    m, n = qr.shape
+    q = np.empty((m, m))
+    lib().orc_qr_q(m, n, _d(qr), _d(_c(rdiag)), _d(q))
+    return q
+
+
+def _qr_solve(fn, qr, rdiag, b):
+    qr, b = _c(qr), _c(b)
+    if b.ndim == 1:
+        b = b.reshape(-1, 1)
+    m, n = qr.shape
+    x = np.empty((max(m, n), b.shape[1]))
+    st = fn(m, n, _d(qr), _d(_c(rdiag)), b.shape[0], b.shape[1], _d(b), _d(x))
+    return st, x
+
+
+def qr_solve(qr, rdiag, b):
+    """The least-squares solve QRDecomposition::solve means (qr.rs:191-232 with
+    the reflections of qr.rs:75-104): n x nx, or the status on failure."""
+    st, x = _qr_solve(lib().orc_qr_solve, qr, rdiag, b)
+    if st:
+        raise OracleError(st)
+    return x[:qr.shape[1]]
+
+
+def qr_solve_literal(qr, rdiag, b):
+    """QRDecomposition::solve exactly as written (qr.rs:191-232; m == n)."""
+    st, x = _qr_solve(lib().orc_qr_solve_literal, qr, rdiag, b)
+    if st:
+        raise OracleError(st)
+    return x[:qr.shape[1]]
+
+
+def pinverse(a):
+    """Matrix::pinverse (src/matrix/mod.rs:1049-1057): n x m."""
+    a = _c(a)
+    m, n = a.shape
+    p = np.empty((n, m))
+    st = lib().orc_pinverse(m, n, _d(a), _d(p))
+    if st:
+        raise OracleError(st)
+    return p

@@ -27,7 +27,7 @@ int ws_get(la_ctx *ctx, int id, size_t bytes, void **out) {
     if (bytes == 0) bytes = 256;
     if (w.size < bytes) {
         size_t total = bytes;
-        for (int i = 0; i < 8; ++i)
+        for (int i = 0; i < WS_COUNT; ++i)
             if (i != id) total += ctx->ws[i].size;
         if (ctx->ws_limit > 0 && (int64_t)total > ctx->ws_limit)
             return set_error(ctx, LA_ERR_NOMEM, "workspace limit exceeded");
@@ -254,6 +254,9 @@ size_t rounded(int64_t count, size_t elem) {
     return (((size_t)std::max<int64_t>(count, 1) * elem) + 255) & ~(size_t)255;
 }
 
+}  // namespace
+
+namespace lahip {
 int read_scratch(la_ctx *ctx, const int32_t *d, int32_t *h, int n) {
     LA_HIP_CHECK(ctx, hipMemcpyAsync(ctx->h_pinned, d, n * sizeof(int32_t),
                                      hipMemcpyDeviceToHost, ctx->stream));
@@ -274,7 +277,8 @@ int check_sticky(la_ctx *ctx) {
     return LA_OK;
 }
 
-}  // namespace
+}  // namespace lahip
+
 
 // ------------------------------------------------------------------ GEMM
 extern "C" int la_dgemm_dev(la_ctx *ctx, int64_t m, int64_t n, int64_t k, double alpha,

@@ -71,7 +71,7 @@ struct la_ctx {
     int64_t hbm_bytes = 0;
     int clock_khz = 0;
     int64_t ws_limit = 0;
-    lahip::Workspace ws[8];     // independent grow-only workspaces (WS_* ids)
+    lahip::Workspace ws[9];     // independent grow-only workspaces (WS_* ids)
     int32_t *d_scratch = nullptr;   // 4 KiB of device status/counter words
     int32_t *h_pinned = nullptr;    // 4 KiB pinned host mirror
     std::string last_error;
@@ -96,7 +96,7 @@ namespace lahip {
 // invalidates earlier pointers from the SAME slot, so each slot has one user
 // per API call.
 enum { WS_HOST = 0, WS_PANEL = 1, WS_SYNC = 2, WS_PIV = 3, WS_LP = 4, WS_IR = 5,
-       WS_BATCH = 6, WS_GEN = 7 };
+       WS_BATCH = 6, WS_GEN = 7, WS_QR = 8, WS_COUNT = 9 };
 
 // Device scratch words (ctx->d_scratch, 1024 int32):
 //   [0]  sticky device error (LA_ERR_TIMEOUT from bounded waits)
@@ -105,6 +105,10 @@ enum { SCR_ERR = 0, SCR_CHOL = 8, SCR_SING = 16, SCR_STATUS = 24, SCR_RED = 64 }
 
 int set_error(la_ctx *ctx, int status, const char *what);
 int hip_fail(la_ctx *ctx, hipError_t e, const char *what);
+// Copies n device status words to the host (synchronous).
+int read_scratch(la_ctx *ctx, const int32_t *d, int32_t *h, int n);
+// LA_OK, or the sticky device error of a bounded wait (then cleared).
+int check_sticky(la_ctx *ctx);
 // Returns a device buffer of at least `bytes` from workspace slot `id`.
 int ws_get(la_ctx *ctx, int id, size_t bytes, void **out);
 

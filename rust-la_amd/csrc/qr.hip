@@ -1,0 +1,598 @@
+// Householder QR (QRDecomposition, src/decomp/qr.rs:26-232) and the
+// pseudo-inverse (Matrix::pinverse, src/matrix/mod.rs:1049-1057).
+//
+// The factor is the reference's: for each column c < min(m, n),
+//   a_c = -sign(x_cc) |x|  (+|x| when x_cc <= 0),   x = A[c.., c],
+//   u = x - a_c e_c stored in place (A[c][c] = x_cc - a_c),  rdiag[c] = a_c,
+//   every later column y -> y + <y,u>/(a_c u_c) u  (the reflection H_c),
+// skipped when a_c = 0.  It is computed blocked, 32 columns per panel:
+//   * inside the panel, column by column with two small launches per column
+//     (the dot products of u with the other panel columns, then their
+//     update fused with the next column's partial norm), every reduction in
+//     a fixed order so that the result is the same in every run;
+//   * the panel's reflections as one block, H_c H_c+1 ... = I - V T V^T
+//     (T upper triangular, T_jj = tau_j = -1/(a_j u_j), forward column-wise
+//     accumulation of Schreiber and Van Loan), applied to the trailing
+//     columns as three MFMA products, A2 -= V (T^T (V^T A2)).
+// get_q applies the blocks to the partial identity in reverse order
+// (Q = (I - V T V^T) Q), the least-squares solve applies them to B in
+// forward order and finishes with R x = (Q^T B)[0..n).
+#include "la_common.hpp"
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+namespace lahip {
+
+constexpr int QR_NB = 32;       // panel width
+constexpr int QR_PARTS = 256;   // max workgroups of the column kernels
+
+// Row r of the panel's current column is handled by thread (r - r0) of the
+// grid (rows beyond the grid's size wrap by the grid size).
+
+// Partial sums of x^2 over rows [r0, m) of column c.
+__global__ __launch_bounds__(256)
+void k_qr_norm_partial(int m, const double *__restrict__ A, int64_t lda, int c,
+                       double *__restrict__ part)
+{
+    __shared__ double red[256];
+    const int t = threadIdx.x;
+    const int stride = gridDim.x * 256;
+    double s = 0.0;
+    for (int r = c + blockIdx.x * 256 + t; r < m; r += stride) {
+        const double x = A[(int64_t)r * lda + c];
+        s = fma(x, x, s);
+    }
+    red[t] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (t < h) red[t] += red[t + h];
+        __syncthreads();
+    }
+    if (t == 0) part[blockIdx.x] = red[0];
+}
+
+// Sum of the P partials in a fixed tree order; every workgroup gets the same
+// value.  Needs a 256-entry LDS buffer; all threads receive the sum.
+__device__ inline double tree_sum(const double *__restrict__ part, int P, double *red)
+{
+    const int t = threadIdx.x;
+    red[t] = t < P ? part[t] : 0.0;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (t < h) red[t] += red[t + h];
+        __syncthreads();
+    }
+    const double s = red[0];
+    __syncthreads();
+    return s;
+}
+
+// a_c, u_c from the column's norm (every workgroup, same order), then the
+// partial dot products <u, A[., col]> for the panel columns col in (c, cend).
+// Workgroup 0 stores (a_c, u_c) in scal[2c..2c+1].
+__global__ __launch_bounds__(256)
+void k_qr_dots(int m, const double *__restrict__ A, int64_t lda, int c, int cend,
+               const double *__restrict__ npart, int P, double *__restrict__ scal,
+               double *__restrict__ dpart)
+{
+    __shared__ double red[256];
+    __shared__ double dsum[8][QR_NB];
+    const int t = threadIdx.x;
+    const double nrm2 = tree_sum(npart, P, red);
+    const double x0 = A[(int64_t)c * lda + c];
+    const double av = x0 > 0.0 ? -sqrt(nrm2) : sqrt(nrm2);
+    const double u0 = x0 - av;
+    if (blockIdx.x == 0 && t == 0) { scal[2 * c] = av; scal[2 * c + 1] = u0; }
+    const int ncol = cend - c - 1;
+    if (av == 0.0 || ncol <= 0) return;
+    double acc[QR_NB - 1];
+#pragma unroll
+    for (int j = 0; j < QR_NB - 1; ++j) acc[j] = 0.0;
+    const int stride = gridDim.x * 256;
+    for (int r = c + blockIdx.x * 256 + t; r < m; r += stride) {
+        const double *row = A + (int64_t)r * lda;
+        const double u = (r == c) ? u0 : row[c];
+#pragma unroll
+        for (int j = 0; j < QR_NB - 1; ++j)
+            if (j < ncol) acc[j] = fma(u, row[c + 1 + j], acc[j]);
+    }
+    // workgroup sum per column: eight groups of 32 lanes in a fixed order
+    // (j is static in the unrolled loop, ncol is the same in every thread)
+    const int g = t >> 5, l = t & 31;
+#pragma unroll
+    for (int j = 0; j < QR_NB - 1; ++j) {
+        if (j < ncol) {
+            red[t] = acc[j];
+            __syncthreads();
+            if (l == 0) {
+                double s = 0.0;
+                for (int q = 0; q < 32; ++q) s += red[32 * g + q];
+                dsum[g][j] = s;
+            }
+            __syncthreads();
+        }
+    }
+    if (t < ncol) {
+        double s = 0.0;
+        for (int q = 0; q < 8; ++q) s += dsum[q][t];
+        dpart[(int64_t)blockIdx.x * QR_NB + t] = s;
+    }
+}
+
+// y += <y,u>/(a_c u_c) u for the panel columns in (c, cend), then the partial
+// norm of column c + 1 over rows > c (if c + 1 < cend), and A[c][c] = u_c,
+// rdiag[c] = a_c.  The dot products are summed in the same fixed order by
+// every workgroup.
+__global__ __launch_bounds__(256)
+void k_qr_apply(int m, double *__restrict__ A, int64_t lda, int c, int cend,
+                const double *__restrict__ dpart, int P, const double *__restrict__ scal,
+                double *__restrict__ rdiag, double *__restrict__ npart)
+{
+    __shared__ double fac[QR_NB];
+    __shared__ double red[256];
+    const int t = threadIdx.x;
+    const double av = scal[2 * c], u0 = scal[2 * c + 1];
+    const int ncol = cend - c - 1;
+    if (t < QR_NB) {
+        double f = 0.0;
+        if (av != 0.0 && t < ncol) {
+            double s = 0.0;
+            for (int b = 0; b < P; ++b) s += dpart[(int64_t)b * QR_NB + t];
+            f = s / (av * u0);
+        }
+        fac[t] = f;
+    }
+    __syncthreads();
+    const int stride = gridDim.x * 256;
+    double nrm = 0.0;
+    for (int r = c + blockIdx.x * 256 + t; r < m; r += stride) {
+        double *row = A + (int64_t)r * lda;
+        const double u = (r == c) ? u0 : row[c];
+        if (av != 0.0) {
+            for (int j = 0; j < ncol; ++j) {
+                const double y = fma(fac[j], u, row[c + 1 + j]);
+                row[c + 1 + j] = y;
+                if (j == 0 && r > c) nrm = fma(y, y, nrm);
+            }
+        } else if (ncol > 0 && r > c) {
+            const double y = row[c + 1];
+            nrm = fma(y, y, nrm);
+        }
+    }
+    if (blockIdx.x == 0 && t == 0) {
+        rdiag[c] = av;
+        if (av != 0.0) A[(int64_t)c * lda + c] = u0;
+    }
+    if (ncol > 0) {
+        red[t] = nrm;
+        __syncthreads();
+        for (int h = 128; h > 0; h >>= 1) {
+            if (t < h) red[t] += red[t + h];
+            __syncthreads();
+        }
+        if (t == 0) npart[blockIdx.x] = red[0];
+    }
+}
+
+// V (rows x nb, ld nb) and its transpose Vt (nb x rows, ld rows) of the
+// panel's Householder vectors: column j is zero above row j, u_j at row j
+// (already stored in A), A below; columns whose a_j = 0 are zero.
+__global__ __launch_bounds__(256)
+void k_qr_make_v(int rows, int nb, const double *__restrict__ A, int64_t lda,
+                 const double *__restrict__ rd, double *__restrict__ V, double *__restrict__ Vt)
+{
+    const int64_t e = blockIdx.x * (int64_t)256 + threadIdx.x;
+    if (e >= (int64_t)rows * nb) return;
+    const int r = (int)(e / nb), j = (int)(e - (int64_t)r * nb);
+    double v = 0.0;
+    if (r >= j && rd[j] != 0.0) v = A[(int64_t)r * lda + j];
+    V[(int64_t)r * nb + j] = v;
+    Vt[(int64_t)j * rows + r] = v;
+}
+
+// T (nb x nb, upper) of I - V T V^T = H_0 ... H_nb-1 from the Gram matrix
+// G = V^T V: T_jj = tau_j, T[0:j, j] = -tau_j T[0:j, 0:j] G[0:j, j]
+// (tau_j = -1/(a_j u_j), 0 for a skipped column).  One workgroup; also
+// writes Tt = T^T.
+__global__ __launch_bounds__(64)
+void k_qr_larft(int nb, const double *__restrict__ G, const double *__restrict__ rd,
+                const double *__restrict__ diagu, int64_t ldu, double *__restrict__ T,
+                double *__restrict__ Tt)
+{
+    __shared__ double sT[QR_NB][QR_NB];
+    __shared__ double w[QR_NB];
+    const int t = threadIdx.x;
+    for (int e = t; e < QR_NB * QR_NB; e += 64) sT[e / QR_NB][e % QR_NB] = 0.0;
+    __syncthreads();
+    for (int j = 0; j < nb; ++j) {
+        const double a = rd[j], u = diagu[(int64_t)j * ldu + j];
+        const double tau = (a != 0.0) ? -1.0 / (a * u) : 0.0;
+        // w = T[0:j, 0:j] G[0:j, j]
+        if (t < j) {
+            double s = 0.0;
+            for (int q = t; q < j; ++q) s += sT[t][q] * G[q * nb + j];
+            w[t] = s;
+        }
+        __syncthreads();
+        if (t < j) sT[t][j] = -tau * w[t];
+        if (t == 0) sT[j][j] = tau;
+        __syncthreads();
+    }
+    for (int e = t; e < nb * nb; e += 64) {
+        const int i = e / nb, j = e % nb;
+        T[e] = sT[i][j];
+        Tt[e] = sT[j][i];
+    }
+}
+
+// Q = partial identity (ones at (i, i) for i < dc), m x m, ld ldq.
+__global__ __launch_bounds__(256)
+void k_qr_partial_identity(int m, int dc, double *__restrict__ Q, int64_t ldq)
+{
+    const int64_t e = blockIdx.x * (int64_t)256 + threadIdx.x;
+    if (e >= (int64_t)m * m) return;
+    const int i = (int)(e / m), j = (int)(e - (int64_t)i * m);
+    Q[(int64_t)i * ldq + j] = (i == j && i < dc) ? 1.0 : 0.0;
+}
+
+// R (rows x cols, ld ldr) from the packed factor: strict upper part of qr,
+// rdiag on the diagonal, zero below; with Rt (cols x rows) its transpose.
+__global__ __launch_bounds__(256)
+void k_qr_make_r(int rows, int cols, const double *__restrict__ QR, int64_t ldq,
+                 const double *__restrict__ rd, double *__restrict__ R, int64_t ldr,
+                 double *__restrict__ Rt)
+{
+    const int64_t e = blockIdx.x * (int64_t)256 + threadIdx.x;
+    if (e >= (int64_t)rows * cols) return;
+    const int i = (int)(e / cols), j = (int)(e - (int64_t)i * cols);
+    const double v = i < j ? QR[(int64_t)i * ldq + j] : (i == j ? rd[i] : 0.0);
+    R[(int64_t)i * ldr + j] = v;
+    if (Rt) Rt[(int64_t)j * rows + i] = v;
+}
+
+// B (rows x cols, ld ldb) -> X (ld ldx)
+__global__ __launch_bounds__(256)
+void k_qr_copy(int rows, int cols, const double *__restrict__ B, int64_t ldb,
+               double *__restrict__ X, int64_t ldx)
+{
+    const int64_t e = blockIdx.x * (int64_t)256 + threadIdx.x;
+    if (e >= (int64_t)rows * cols) return;
+    const int i = (int)(e / cols), j = (int)(e - (int64_t)i * cols);
+    X[(int64_t)i * ldx + j] = B[(int64_t)i * ldb + j];
+}
+
+namespace {
+
+// Workspace of one QR call.
+struct QrWs {
+    double *V, *Vt, *G, *T, *Tt, *W, *W2, *npart, *dpart, *scal;
+};
+
+int qr_ws(la_ctx *ctx, int64_t m, int64_t n, int64_t wcols, QrWs &w)
+{
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const int64_t dc = std::min(m, n);
+    const size_t bv = al((size_t)m * QR_NB * 8), bg = al((size_t)QR_NB * QR_NB * 8),
+                 bw = al((size_t)QR_NB * std::max<int64_t>(wcols, 1) * 8),
+                 bp = al((size_t)QR_PARTS * 8), bd = al((size_t)QR_PARTS * QR_NB * 8),
+                 bs = al((size_t)2 * std::max<int64_t>(dc, 1) * 8);
+    void *base = nullptr;
+    int st = ws_get(ctx, WS_QR, 2 * bv + 3 * bg + 2 * bw + bp + bd + bs, &base);
+    if (st) return st;
+    char *p = (char *)base;
+    w.V = (double *)p; p += bv;
+    w.Vt = (double *)p; p += bv;
+    w.G = (double *)p; p += bg;
+    w.T = (double *)p; p += bg;
+    w.Tt = (double *)p; p += bg;
+    w.W = (double *)p; p += bw;
+    w.W2 = (double *)p; p += bw;
+    w.npart = (double *)p; p += bp;
+    w.dpart = (double *)p; p += bd;
+    w.scal = (double *)p;
+    return LA_OK;
+}
+
+int grid_rows(int64_t rows) { return (int)std::min<int64_t>(QR_PARTS, std::max<int64_t>(1, ceil_div(rows, 256))); }
+
+// V, Vt and T of the panel at column k0 (width nb) of the packed factor.
+int qr_block(la_ctx *ctx, int64_t m, const double *QR, int64_t ldq, const double *rdiag,
+             int64_t k0, int nb, QrWs &w)
+{
+    const int64_t rows = m - k0;
+    hipLaunchKernelGGL(k_qr_make_v, dim3((unsigned)ceil_div(rows * nb, 256)), dim3(256), 0,
+                       ctx->stream, (int)rows, nb, QR + k0 * ldq + k0, ldq, rdiag + k0, w.V, w.Vt);
+    LA_HIP_CHECK(ctx, hipGetLastError());
+    int st = gemm_rm<double>(ctx, nb, nb, rows, 1.0, w.Vt, rows, w.V, nb, 0.0, w.G, nb, false,
+                             false);
+    if (st) return st;
+    hipLaunchKernelGGL(k_qr_larft, dim3(1), dim3(64), 0, ctx->stream, nb, (const double *)w.G,
+                       rdiag + k0, QR + k0 * ldq + k0, ldq, w.T, w.Tt);
+    LA_HIP_CHECK(ctx, hipGetLastError());
+    return LA_OK;
+}
+
+// C (rows x cols, ld ldc, the rows k0.. of the operand) <- (I - V op(T) V^T) C
+// with op(T) = T^T (trans) or T.
+int qr_apply_block(la_ctx *ctx, int64_t rows, int nb, int64_t cols, double *C, int64_t ldc,
+                   bool trans, QrWs &w)
+{
+    if (cols <= 0) return LA_OK;
+    int st = gemm_rm<double>(ctx, nb, cols, rows, 1.0, w.Vt, rows, C, ldc, 0.0, w.W, cols, false,
+                             false);
+    if (st) return st;
+    st = gemm_rm<double>(ctx, nb, cols, nb, 1.0, trans ? w.Tt : w.T, nb, w.W, cols, 0.0, w.W2,
+                         cols, false, false);
+    if (st) return st;
+    return gemm_rm<double>(ctx, rows, cols, nb, -1.0, w.V, nb, w.W2, cols, 1.0, C, ldc, false,
+                           false);
+}
+
+}  // namespace
+
+// In-place factor of A (m x n, ld lda); rdiag (device, min(m, n)).
+int geqrf_device(la_ctx *ctx, int64_t m, int64_t n, double *A, int64_t lda, double *rdiag)
+{
+    if (m > INT32_MAX / 2 || n > INT32_MAX / 2) return set_error(ctx, LA_ERR_ARG, "qr: extent too large");
+    const int64_t dc = std::min(m, n);
+    QrWs w;
+    int st = qr_ws(ctx, m, n, n, w);
+    if (st) return st;
+    for (int64_t k0 = 0; k0 < dc; k0 += QR_NB) {
+        const int nb = (int)std::min<int64_t>(QR_NB, dc - k0);
+        const int64_t cend = k0 + nb;
+        // column by column inside the panel; the first column's norm first
+        int P = grid_rows(m - k0);
+        hipLaunchKernelGGL(k_qr_norm_partial, dim3(P), dim3(256), 0, ctx->stream, (int)m,
+                           (const double *)A, lda, (int)k0, w.npart);
+        LA_HIP_CHECK(ctx, hipGetLastError());
+        for (int64_t c = k0; c < cend; ++c) {
+            // the grid of both launches covers rows [c, m); its size also
+            // fixes the number of partials the next step reduces
+            const int Pc = grid_rows(m - c);
+            hipLaunchKernelGGL(k_qr_dots, dim3(Pc), dim3(256), 0, ctx->stream, (int)m,
+                               (const double *)A, lda, (int)c, (int)cend, (const double *)w.npart,
+                               P, w.scal, w.dpart);
+            hipLaunchKernelGGL(k_qr_apply, dim3(Pc), dim3(256), 0, ctx->stream, (int)m, A, lda,
+                               (int)c, (int)cend, (const double *)w.dpart, Pc,
+                               (const double *)w.scal, rdiag, w.npart);
+            LA_HIP_CHECK(ctx, hipGetLastError());
+            P = Pc;
+        }
+        // the block's reflections on the trailing columns
+        if (cend < n) {
+            if ((st = qr_block(ctx, m, A, lda, rdiag, k0, nb, w))) return st;
+            if ((st = qr_apply_block(ctx, m - k0, nb, n - cend, A + k0 * lda + cend, lda, true, w)))
+                return st;
+        }
+    }
+    return LA_OK;
+}
+
+// Q (m x m, ld ldq) of the packed factor QR (m x n, ld ldqr).
+int qr_q_device(la_ctx *ctx, int64_t m, int64_t n, const double *QR, int64_t ldqr,
+                const double *rdiag, double *Q, int64_t ldq)
+{
+    const int64_t dc = std::min(m, n);
+    QrWs w;
+    int st = qr_ws(ctx, m, n, m, w);
+    if (st) return st;
+    hipLaunchKernelGGL(k_qr_partial_identity, dim3((unsigned)ceil_div(m * m, 256)), dim3(256), 0,
+                       ctx->stream, (int)m, (int)dc, Q, ldq);
+    LA_HIP_CHECK(ctx, hipGetLastError());
+    const int64_t last = ((dc - 1) / QR_NB) * QR_NB;
+    for (int64_t k0 = last; k0 >= 0; k0 -= QR_NB) {
+        const int nb = (int)std::min<int64_t>(QR_NB, dc - k0);
+        if ((st = qr_block(ctx, m, QR, ldqr, rdiag, k0, nb, w))) return st;
+        if ((st = qr_apply_block(ctx, m - k0, nb, m, Q + k0 * ldq, ldq, false, w))) return st;
+    }
+    return LA_OK;
+}
+
+// X (n x nx) least-squares solution; Y (m x nx, ld ldy) holds B on entry and
+// Q^T B on exit.  Full rank and m >= n checked by the caller.
+int qr_solve_device(la_ctx *ctx, int64_t m, int64_t n, const double *QR, int64_t ldqr,
+                    const double *rdiag, int64_t nx, double *Y, int64_t ldy, double *R,
+                    double *X, int64_t ldx)
+{
+    QrWs w;
+    int st = qr_ws(ctx, m, n, nx, w);
+    if (st) return st;
+    for (int64_t k0 = 0; k0 < n; k0 += QR_NB) {
+        const int nb = (int)std::min<int64_t>(QR_NB, n - k0);
+        if ((st = qr_block(ctx, m, QR, ldqr, rdiag, k0, nb, w))) return st;
+        if ((st = qr_apply_block(ctx, m - k0, nb, nx, Y + k0 * ldy, ldy, true, w))) return st;
+    }
+    // R x = y[0..n): R with rdiag on its diagonal, solved as an upper factor
+    hipLaunchKernelGGL(k_qr_make_r, dim3((unsigned)ceil_div(n * n, 256)), dim3(256), 0,
+                       ctx->stream, (int)n, (int)n, QR, ldqr, rdiag, R, n, (double *)nullptr);
+    hipLaunchKernelGGL(k_qr_copy, dim3((unsigned)ceil_div(n * nx, 256)), dim3(256), 0,
+                       ctx->stream, (int)n, (int)nx, (const double *)Y, ldy, X, ldx);
+    LA_HIP_CHECK(ctx, hipGetLastError());
+    if (nx >= TRSM_MIN_RHS && n >= TRSM_MIN_N) return trsm_blocked(ctx, 'U', n, R, n, nx, X, ldx);
+    return trsv_blocked<double>(ctx, 'U', false, false, n, R, n, nx, X, ldx);
+}
+
+}  // namespace lahip
+
+using namespace lahip;
+
+namespace {
+struct Bufs {
+    char *base;
+    size_t off = 0;
+    template <typename T> T *take(int64_t count) {
+        T *p = reinterpret_cast<T *>(base + off);
+        off += (((size_t)std::max<int64_t>(count, 1) * sizeof(T)) + 255) & ~(size_t)255;
+        return p;
+    }
+};
+size_t rbytes(int64_t count, size_t elem)
+{
+    return (((size_t)std::max<int64_t>(count, 1) * elem) + 255) & ~(size_t)255;
+}
+// rdiag -> full-rank flag as QRDecomposition::is_full_rank (qr.rs:109-116)
+// sees it: LA_ERR_SHAPE when n > m (it reads rdiag past its end, a panic),
+// LA_SINGULAR when some rdiag[j] == 0, else LA_OK.
+int rank_status(la_ctx *ctx, int64_t m, int64_t n, const double *d_rdiag)
+{
+    if (n > m) return set_error(ctx, LA_ERR_SHAPE, "qr is_full_rank: index out of bounds (cols > rows)");
+    std::vector<double> h((size_t)n);
+    LA_HIP_CHECK(ctx, hipMemcpyAsync(h.data(), d_rdiag, n * 8, hipMemcpyDeviceToHost, ctx->stream));
+    LA_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    for (int64_t j = 0; j < n; ++j)
+        if (h[(size_t)j] == 0.0) return LA_SINGULAR;
+    return LA_OK;
+}
+}  // namespace
+
+extern "C" int la_dgeqrf_dev(la_ctx *ctx, int64_t m, int64_t n, double *A, int64_t lda,
+                             double *d_rdiag)
+{
+    if (!ctx || !A || !d_rdiag) return LA_ERR_ARG;
+    if (m <= 0 || n <= 0) return set_error(ctx, LA_ERR_SHAPE, "qr: empty matrix");
+    if (lda < n) return LA_ERR_ARG;
+    return geqrf_device(ctx, m, n, A, lda, d_rdiag);
+}
+
+extern "C" int la_dgeqrf_rm(la_ctx *ctx, int64_t m, int64_t n, const double *A, double *QR,
+                            double *rdiag)
+{
+    if (!ctx) return LA_ERR_ARG;
+    if (m <= 0 || n <= 0) return set_error(ctx, LA_ERR_SHAPE, "qr: empty matrix");  // mod.rs:209
+    if (!A || !QR || !rdiag) return LA_ERR_ARG;
+    const int64_t dc = std::min(m, n);
+    void *base = nullptr;
+    int st = ws_get(ctx, WS_HOST, rbytes(m * n, 8) + rbytes(dc, 8), &base);
+    if (st) return st;
+    Bufs b{(char *)base};
+    double *dA = b.take<double>(m * n), *dr = b.take<double>(dc);
+    LA_HIP_CHECK(ctx, hipMemcpyAsync(dA, A, m * n * 8, hipMemcpyHostToDevice, ctx->stream));
+    if ((st = geqrf_device(ctx, m, n, dA, n, dr))) return st;
+    LA_HIP_CHECK(ctx, hipMemcpyAsync(QR, dA, m * n * 8, hipMemcpyDeviceToHost, ctx->stream));
+    LA_HIP_CHECK(ctx, hipMemcpyAsync(rdiag, dr, dc * 8, hipMemcpyDeviceToHost, ctx->stream));
+    LA_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return check_sticky(ctx);
+}
+
+extern "C" int la_dqr_q_dev(la_ctx *ctx, int64_t m, int64_t n, const double *QR, int64_t ldqr,
+                            const double *d_rdiag, double *Q, int64_t ldq)
+{
+    if (!ctx || !QR || !d_rdiag || !Q) return LA_ERR_ARG;
+    if (m <= 0 || n <= 0) return set_error(ctx, LA_ERR_SHAPE, "qr: empty matrix");
+    if (ldqr < n || ldq < m) return LA_ERR_ARG;
+    return qr_q_device(ctx, m, n, QR, ldqr, d_rdiag, Q, ldq);
+}
+
+extern "C" int la_dqr_q_rm(la_ctx *ctx, int64_t m, int64_t n, const double *QR,
+                           const double *rdiag, double *Q)
+{
+    if (!ctx) return LA_ERR_ARG;
+    if (m <= 0 || n <= 0) return set_error(ctx, LA_ERR_SHAPE, "qr: empty matrix");
+    if (!QR || !rdiag || !Q) return LA_ERR_ARG;
+    const int64_t dc = std::min(m, n);
+    void *base = nullptr;
+    int st = ws_get(ctx, WS_HOST, rbytes(m * n, 8) + rbytes(dc, 8) + rbytes(m * m, 8), &base);
+    if (st) return st;
+    Bufs b{(char *)base};
+    double *dqr = b.take<double>(m * n), *dr = b.take<double>(dc), *dq = b.take<double>(m * m);
+    LA_HIP_CHECK(ctx, hipMemcpyAsync(dqr, QR, m * n * 8, hipMemcpyHostToDevice, ctx->stream));
+    LA_HIP_CHECK(ctx, hipMemcpyAsync(dr, rdiag, dc * 8, hipMemcpyHostToDevice, ctx->stream));
+    if ((st = qr_q_device(ctx, m, n, dqr, n, dr, dq, m))) return st;
+    LA_HIP_CHECK(ctx, hipMemcpyAsync(Q, dq, m * m * 8, hipMemcpyDeviceToHost, ctx->stream));
+    LA_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return check_sticky(ctx);
+}
+
+extern "C" int la_dqrs_dev(la_ctx *ctx, int64_t m, int64_t n, const double *QR, int64_t ldqr,
+                           const double *d_rdiag, int64_t nx, const double *B, int64_t ldb,
+                           double *X, int64_t ldx)
+{
+    if (!ctx || !QR || !d_rdiag || !B || !X) return LA_ERR_ARG;
+    if (m <= 0 || n <= 0 || nx <= 0) return set_error(ctx, LA_ERR_SHAPE, "qr solve: empty");
+    if (ldqr < n || ldb < nx || ldx < nx) return LA_ERR_ARG;
+    int st = rank_status(ctx, m, n, d_rdiag);
+    if (st) return st;
+    void *base = nullptr;
+    if ((st = ws_get(ctx, WS_HOST, rbytes(m * nx, 8) + rbytes(n * n, 8), &base))) return st;
+    Bufs b{(char *)base};
+    double *Y = b.take<double>(m * nx), *R = b.take<double>(n * n);
+    hipLaunchKernelGGL(k_qr_copy, dim3((unsigned)ceil_div(m * nx, 256)), dim3(256), 0, ctx->stream,
+                       (int)m, (int)nx, B, ldb, Y, nx);
+    LA_HIP_CHECK(ctx, hipGetLastError());
+    return qr_solve_device(ctx, m, n, QR, ldqr, d_rdiag, nx, Y, nx, R, X, ldx);
+}
+
+extern "C" int la_dqrs_rm(la_ctx *ctx, int64_t m, int64_t n, const double *QR,
+                          const double *rdiag, int64_t b_rows, int64_t nx, const double *B,
+                          double *X)
+{
+    if (!ctx) return LA_ERR_ARG;
+    if (b_rows != m) return set_error(ctx, LA_ERR_SHAPE, "qr solve: b.rows() != rows");  // qr.rs:193
+    if (m <= 0 || n <= 0 || nx <= 0) return set_error(ctx, LA_ERR_SHAPE, "qr solve: empty");
+    if (!QR || !rdiag || !B || !X) return LA_ERR_ARG;
+    if (n > m) return set_error(ctx, LA_ERR_SHAPE, "qr is_full_rank: index out of bounds (cols > rows)");
+    for (int64_t j = 0; j < n; ++j)
+        if (rdiag[j] == 0.0) return LA_SINGULAR;                   // qr.rs:194-196: None
+    const int64_t dc = n;
+    void *base = nullptr;
+    int st = ws_get(ctx, WS_HOST,
+                    rbytes(m * n, 8) + rbytes(dc, 8) + rbytes(m * nx, 8) + rbytes(n * n, 8) +
+                        rbytes(n * nx, 8),
+                    &base);
+    if (st) return st;
+    Bufs b{(char *)base};
+    double *dqr = b.take<double>(m * n), *dr = b.take<double>(dc), *Y = b.take<double>(m * nx);
+    double *R = b.take<double>(n * n), *dX = b.take<double>(n * nx);
+    LA_HIP_CHECK(ctx, hipMemcpyAsync(dqr, QR, m * n * 8, hipMemcpyHostToDevice, ctx->stream));
+    LA_HIP_CHECK(ctx, hipMemcpyAsync(dr, rdiag, dc * 8, hipMemcpyHostToDevice, ctx->stream));
+    LA_HIP_CHECK(ctx, hipMemcpyAsync(Y, B, m * nx * 8, hipMemcpyHostToDevice, ctx->stream));
+    if ((st = qr_solve_device(ctx, m, n, dqr, n, dr, nx, Y, nx, R, dX, nx))) return st;
+    LA_HIP_CHECK(ctx, hipMemcpyAsync(X, dX, n * nx * 8, hipMemcpyDeviceToHost, ctx->stream));
+    LA_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return check_sticky(ctx);
+}
+
+extern "C" int la_dpinverse_rm(la_ctx *ctx, int64_t m, int64_t n, const double *A, double *P)
+{
+    if (!ctx) return LA_ERR_ARG;
+    if (m <= 0 || n <= 0) return set_error(ctx, LA_ERR_SHAPE, "pinverse: empty matrix");
+    if (!A || !P) return LA_ERR_ARG;
+    const int64_t dc = std::min(m, n);
+    void *base = nullptr;
+    int st = ws_get(ctx, WS_HOST,
+                    2 * rbytes(m * n, 8) + rbytes(dc, 8) + 2 * rbytes(m * n, 8) +
+                        2 * rbytes(n * n, 8) + rbytes(n, 8) + rbytes(4, 4) + rbytes(n * m, 8),
+                    &base);
+    if (st) return st;
+    Bufs b{(char *)base};
+    double *dA = b.take<double>(m * n), *dqr = b.take<double>(m * n), *dr = b.take<double>(dc);
+    double *R = b.take<double>(m * n), *Rt = b.take<double>(m * n);
+    double *G = b.take<double>(n * n), *Gi = b.take<double>(n * n);
+    uint64_t *piv = b.take<uint64_t>(n);
+    int32_t *info = b.take<int32_t>(4);
+    double *dP = b.take<double>(n * m);
+    LA_HIP_CHECK(ctx, hipMemcpyAsync(dA, A, m * n * 8, hipMemcpyHostToDevice, ctx->stream));
+    LA_HIP_CHECK(ctx, hipMemcpyAsync(dqr, dA, m * n * 8, hipMemcpyDeviceToDevice, ctx->stream));
+    if ((st = geqrf_device(ctx, m, n, dqr, n, dr))) return st;
+    // R = get_r() (m x n), G = R^T R (n x n)
+    hipLaunchKernelGGL(k_qr_make_r, dim3((unsigned)ceil_div(m * n, 256)), dim3(256), 0,
+                       ctx->stream, (int)m, (int)n, (const double *)dqr, n, (const double *)dr, R,
+                       n, Rt);
+    LA_HIP_CHECK(ctx, hipGetLastError());
+    if ((st = gemm_rm<double>(ctx, n, n, m, 1.0, Rt, m, R, n, 0.0, G, n, false, false))) return st;
+    // (R^T R)^-1 by LU and two triangular solves; .unwrap() of None panics
+    if ((st = la_dinverse_dev(ctx, n, G, n, piv, Gi, n, info))) return st;
+    int32_t hinfo[3];
+    LA_HIP_CHECK(ctx, hipMemcpyAsync(hinfo, info, 12, hipMemcpyDeviceToHost, ctx->stream));
+    LA_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    if (hinfo[1] != LA_OK) return set_error(ctx, hinfo[1], "getrf: device wait failed");
+    if (hinfo[2] == LA_SINGULAR)
+        return set_error(ctx, LA_SINGULAR, "pinverse: R^T R is singular (inverse().unwrap() on None)");
+    // P = (R^T R)^-1 A^T
+    if ((st = gemm_rm<double>(ctx, n, m, n, 1.0, Gi, n, dA, n, 0.0, dP, m, true, false))) return st;
+    LA_HIP_CHECK(ctx, hipMemcpyAsync(P, dP, n * m * 8, hipMemcpyDeviceToHost, ctx->stream));
+    LA_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return check_sticky(ctx);
+}

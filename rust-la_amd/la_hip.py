@@ -156,6 +156,13 @@ _SIGNATURES = {
     "la_gen_svd_batch_dev": [_vp, _i64, _vp, _vp, _vp, _vp, ctypes.c_double, ctypes.c_int,
                              ctypes.c_uint64, ctypes.c_uint64, _vp, _vp, _vp],
     "la_dinverse_rm": [_vp, _i64, _i64, _dp, _dp],
+    "la_dgeqrf_rm": [_vp, _i64, _i64, _dp, _dp, _dp],
+    "la_dqr_q_rm": [_vp, _i64, _i64, _dp, _dp, _dp],
+    "la_dqrs_rm": [_vp, _i64, _i64, _dp, _dp, _i64, _i64, _dp, _dp],
+    "la_dpinverse_rm": [_vp, _i64, _i64, _dp, _dp],
+    "la_dgeqrf_dev": [_vp, _i64, _i64, _vp, _i64, _vp],
+    "la_dqr_q_dev": [_vp, _i64, _i64, _vp, _i64, _vp, _vp, _i64],
+    "la_dqrs_dev": [_vp, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _i64],
     "la_dinverse_dev": [_vp, _i64, _vp, _i64, _vp, _vp, _i64, _vp],
     "la_autotune": [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p],
     "la_autotune_load": [_vp, ctypes.c_char_p],
@@ -326,6 +333,59 @@ class Context:
             return None
         self._check(st, "dinverse_rm")
         return out
+
+    def qr_new(self, a):
+        """QRDecomposition::new (qr.rs:26-107) -> (qr, rdiag)."""
+        a = _c(a)
+        m, n = a.shape
+        qr = np.empty_like(a)
+        rdiag = np.empty(min(m, n))
+        self._check(self.lib.la_dgeqrf_rm(self.h, m, n, _d(a), _d(qr), _d(rdiag)), "dgeqrf_rm")
+        return qr, rdiag
+
+    def qr_q(self, qr, rdiag):
+        """QRDecomposition::get_q (qr.rs:149-186), m x m."""
+        qr, rdiag = _c(qr), _c(rdiag)
+        m, n = qr.shape
+        q = np.empty((m, m))
+        self._check(self.lib.la_dqr_q_rm(self.h, m, n, _d(qr), _d(rdiag), _d(q)), "dqr_q_rm")
+        return q
+
+    def qr_solve(self, qr, rdiag, b):
+        """QRDecomposition::solve (qr.rs:191-232) as a least-squares solve:
+        (x or None, status)."""
+        qr, rdiag, b = _c(qr), _c(rdiag), _c(b)
+        if b.ndim == 1:
+            b = b.reshape(-1, 1)
+        m, n = qr.shape
+        x = np.empty((n, b.shape[1]))
+        st = self.lib.la_dqrs_rm(self.h, m, n, _d(qr), _d(rdiag), b.shape[0], b.shape[1], _d(b),
+                                 _d(x))
+        return (x if st == LA_OK else None), st
+
+    def pinverse(self, a):
+        """Matrix::pinverse (mod.rs:1049-1057): (n x m or None, status)."""
+        a = _c(a)
+        m, n = a.shape
+        p = np.empty((n, m))
+        st = self.lib.la_dpinverse_rm(self.h, m, n, _d(a), _d(p))
+        return (p if st == LA_OK else None), st
+
+    def geqrf_dev(self, A, rdiag):
+        m, n = A.shape
+        self._check(self.lib.la_dgeqrf_dev(self.h, m, n, _ptr(A), A.stride(0), _ptr(rdiag)),
+                    "dgeqrf_dev")
+
+    def qr_q_dev(self, QR, rdiag, Q):
+        m, n = QR.shape
+        self._check(self.lib.la_dqr_q_dev(self.h, m, n, _ptr(QR), QR.stride(0), _ptr(rdiag),
+                                          _ptr(Q), Q.stride(0)), "dqr_q_dev")
+
+    def qrs_dev(self, QR, rdiag, B, X):
+        """Returns the status (LA_OK, LA_SINGULAR or LA_ERR_SHAPE)."""
+        m, n = QR.shape
+        return self.lib.la_dqrs_dev(self.h, m, n, _ptr(QR), QR.stride(0), _ptr(rdiag),
+                                    B.shape[1], _ptr(B), B.stride(0), _ptr(X), X.stride(0))
 
     def chol_new(self, a):
         """Returns (L or None, status)."""

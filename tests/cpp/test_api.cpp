@@ -162,6 +162,33 @@ int main() {
         return panics([] { (void)m<D>({{1, 2, 3}, {4, 5, 6}}).is_singular(); });
     });
     run("mod.rs:1567 test_is_non_singular", [] { return m<D>({{2, 6}, {6, 3}}).is_non_singular(); });
+    run("qr.rs:240 qr_test", [] {
+        auto a = m<D>({{12, -51, 4}, {6, 167, -68}, {-4, 24, -41}});
+        la::QRDecomposition<D> qr(a);
+        return approx_eq(qr.get_q() * qr.get_r(), a);
+    });
+    run("qr.rs:247 qr_test_m_over_n", [] {
+        auto a = m<D>({{1, 2}, {3, 4}, {5, 6}});
+        la::QRDecomposition<D> qr(a);
+        return approx_eq(qr.get_q() * qr.get_r(), a);
+    });
+    run("qr.rs:254 qr_test_n_over_m", [] {
+        auto a = m<D>({{1, 2, 3}, {4, 5, 6}});
+        la::QRDecomposition<D> qr(a);
+        return approx_eq(qr.get_q() * qr.get_r(), a);
+    });
+    run("qr.rs:191 solve (least squares, square)", [] {
+        auto a = m<D>({{12, -51, 4}, {6, 167, -68}, {-4, 24, -41}});
+        auto x = la::QRDecomposition<D>(a).solve(m<D>({{1}, {2}, {3}}));
+        return x && approx_eq(a * *x, m<D>({{1}, {2}, {3}}));
+    });
+    run("qr.rs:109 is_full_rank panics when cols > rows", [] {
+        return panics([] { (void)la::QRDecomposition<D>(m<D>({{1, 2, 3}, {4, 5, 6}})).is_full_rank(); });
+    });
+    run("mod.rs:1549 test_pinverse", [] {
+        auto a = m<D>({{1, 2}, {3, 4}, {5, 6}});
+        return approx_eq(a.pinverse() * a, Matrix<D>::id(2, 2));
+    });
     std::printf("%d failure(s)\n", failures);
     return failures == 0 ? 0 : 1;
 }

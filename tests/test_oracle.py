@@ -383,3 +383,95 @@ def test_oracle_lu_colcopy_bit_identical(shape):
     r2 = oracle.lu_new_colcopy(a, want_gap=True)
     assert np.array_equal(r1[0], r2[0]) and np.array_equal(r1[1], r2[1])
     assert r1[2] == r2[2] and r1[3] == r2[3]
+
+
+# --------------------------------------------------------------- QR (qr.rs)
+QR_KATS = [
+    m([12.0, -51.0, 4.0], [6.0, 167.0, -68.0], [-4.0, 24.0, -41.0]),   # qr.rs:240-244
+    m([1.0, 2.0], [3.0, 4.0], [5.0, 6.0]),                               # qr.rs:247-251
+    m([1.0, 2.0, 3.0], [4.0, 5.0, 6.0]),                                 # qr.rs:254-258
+]
+
+
+@pytest.mark.parametrize("k", range(len(QR_KATS)))
+def test_qr_kats(k):
+    """The reference's three QR tests: (get_q() * get_r()).approx_eq(a)."""
+    a = QR_KATS[k]
+    qr, rd = oracle.qr_new(a)
+    assert approx_eq(oracle.mul(oracle.qr_q(qr, rd), oracle.qr_r(qr, rd)), a)
+
+
+def test_qr_against_the_closed_form_of_the_first_kat():
+    """qr.rs:240's matrix has the textbook factor R = [[-14, -21, 14],
+    [0, -175, 70], [0, 0, 35]] with the reference's sign rule (a = -|x| for
+    x_00 = 12 > 0, ...): rdiag and the upper part follow exactly."""
+    qr, rd = oracle.qr_new(QR_KATS[0])
+    assert np.allclose(rd, [-14.0, -175.0, 35.0], rtol=0, atol=1e-12)
+    assert np.allclose(np.triu(qr, 1)[[0, 0, 1], [1, 2, 2]], [-21.0, 14.0, 70.0], atol=1e-12)
+
+
+@pytest.mark.parametrize("shape", [(7, 7), (9, 4), (4, 9), (40, 25)])
+def test_qr_invariants_and_lapack(shape):
+    """Q R = A, Q's first min(m, n) columns orthonormal, R = LAPACK's R up to
+    the sign of each row (the reference fixes the sign by a = -sign(x) |x|),
+    the Householder vectors reproduce the reflections."""
+    a = oracle.normal(31, shape[0] * 100 + shape[1], shape)
+    mm, nn = shape
+    qr, rd = oracle.qr_new(a)
+    q, r = oracle.qr_q(qr, rd), oracle.qr_r(qr, rd)
+    k = min(mm, nn)
+    assert np.abs(q @ r - a).max() < 1e-13 * max(1.0, np.abs(a).max()) * max(shape)
+    assert np.abs(q[:, :k].T @ q[:, :k] - np.eye(k)).max() < 1e-13 * max(shape)
+    assert np.all(q[:, k:] == 0.0)
+    r_lapack = scipy.linalg.qr(a, mode="economic")[1]
+    assert np.allclose(np.abs(r[:k]), np.abs(r_lapack[:k]), atol=1e-12 * np.abs(a).max() * max(shape))
+    # H_c applied by hand: I + u u^T / (a u_c)
+    h = np.eye(mm)
+    for c in range(k):
+        u = np.zeros(mm)
+        u[c:] = qr[c:, c]
+        h = h @ (np.eye(mm) + np.outer(u, u) / (rd[c] * qr[c, c]))
+    assert np.abs(h[:, :k] - q[:, :k]).max() < 1e-12 * max(shape)
+
+
+def test_qr_zero_column_and_rank():
+    """A zero column gives a = 0 and is left as it was (qr.rs:64); the factor
+    is then not of full rank (qr.rs:109-116), and is_full_rank panics (index
+    out of bounds) when n > m."""
+    a = oracle.normal(5, 1, (6, 4))
+    a[:, 2] = 0.0
+    qr, rd = oracle.qr_new(a)
+    # column 2 is zero only above the diagonal after the first two
+    # reflections, so rdiag[2] is the norm of a vector that stays zero
+    assert rd[2] == 0.0 and not oracle.qr_is_full_rank(qr, rd)
+    with pytest.raises(oracle.OracleError):
+        oracle.qr_is_full_rank(*oracle.qr_new(oracle.normal(5, 2, (3, 5))))
+
+
+def test_qr_solve_least_squares_and_the_literal_defect():
+    """The corrected solve is the least-squares solution (numpy lstsq); the
+    reference's literal formula (qr.rs:203-214, JAMA's normalisation applied
+    to unnormalised vectors) is not: its residual on qr.rs:240's matrix is
+    in the thousands."""
+    a = QR_KATS[0]
+    b = m([1.0], [2.0], [3.0])
+    qr, rd = oracle.qr_new(a)
+    x = oracle.qr_solve(qr, rd, b)
+    assert np.abs(a @ x - b).max() < 1e-13
+    xl = oracle.qr_solve_literal(qr, rd, b)
+    assert np.abs(a @ xl - b).max() > 1e3
+    tall = oracle.normal(8, 3, (30, 7))
+    bt = oracle.normal(8, 4, (30, 2))
+    xt = oracle.qr_solve(*oracle.qr_new(tall), bt)
+    assert np.allclose(xt, np.linalg.lstsq(tall, bt, rcond=None)[0], atol=1e-12)
+    with pytest.raises(oracle.OracleError):
+        oracle.qr_solve_literal(*oracle.qr_new(tall), bt)        # Matrix::new assert, m > n
+
+
+def test_pinverse_kat_and_numpy():
+    """mod.rs:1549-1552: a.pinverse() * a ~ id(2); and numpy's pinv for a
+    full-column-rank matrix."""
+    a = m([1.0, 2.0], [3.0, 4.0], [5.0, 6.0])
+    assert approx_eq(oracle.mul(oracle.pinverse(a), a), np.eye(2))
+    b = oracle.normal(9, 9, (20, 6))
+    assert np.allclose(oracle.pinverse(b), np.linalg.pinv(b), atol=1e-12)
