@@ -13,7 +13,8 @@
 //   * the panel's reflections as one block, H_c H_c+1 ... = I - V T V^T
 //     (T upper triangular, T_jj = tau_j = -1/(a_j u_j), forward column-wise
 //     accumulation of Schreiber and Van Loan), applied to the trailing
-//     columns as three MFMA products, A2 -= V (T^T (V^T A2)).
+//     columns as A2 -= V (T^T (V^T A2)): V^T A2 (32 rows, long K) split over
+//     row chunks and summed in chunk order, the other two on MFMA.
 // get_q applies the blocks to the partial identity in reverse order
 // (Q = (I - V T V^T) Q), the least-squares solve applies them to B in
 // forward order and finishes with R x = (Q^T B)[0..n).
@@ -175,12 +176,63 @@ void k_qr_apply(int m, double *__restrict__ A, int64_t lda, int c, int cend,
     }
 }
 
-// V (rows x nb, ld nb) and its transpose Vt (nb x rows, ld rows) of the
-// panel's Householder vectors: column j is zero above row j, u_j at row j
-// (already stored in A), A below; columns whose a_j = 0 are zero.
+// W = V^T C (nb x cols) for a tall V (rows x nb, ld nb) and C (rows x cols,
+// ld ldc): a product with a short output and a long K, split over row chunks
+// (blockIdx.y) whose partial products are summed by k_qr_vtc_reduce in chunk
+// order.  Thread t owns column (t & 127) of a 128-column tile and 16 of the
+// 32 rows of W; V is staged in LDS 64 rows at a time, zero past nb.
+constexpr int VT_COLS = 128, VT_SUB = 64, VT_MAXCHUNK = 16;
+__global__ __launch_bounds__(256)
+void k_qr_vtc_partial(int rows, int nb, int cols, int rch, const double *__restrict__ V,
+                      const double *__restrict__ C, int64_t ldc, double *__restrict__ P)
+{
+    __shared__ double vs[VT_SUB][QR_NB];
+    const int t = threadIdx.x;
+    const int col = blockIdx.x * VT_COLS + (t & 127), jh = (t >> 7) * 16;
+    const int r0 = blockIdx.y * rch, r1 = min(rows, r0 + rch);
+    const bool okc = col < cols;
+    double acc[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) acc[j] = 0.0;
+    for (int rs = r0; rs < r1; rs += VT_SUB) {
+        const int rn = min(VT_SUB, r1 - rs);
+        for (int e = t; e < VT_SUB * QR_NB; e += 256) {
+            const int rr = e / QR_NB, j = e - rr * QR_NB;
+            vs[rr][j] = (rr < rn && j < nb) ? V[(int64_t)(rs + rr) * nb + j] : 0.0;
+        }
+        __syncthreads();
+        const double *cp = C + (int64_t)rs * ldc + (okc ? col : 0);
+        for (int rr = 0; rr < rn; ++rr) {
+            const double c = okc ? cp[(int64_t)rr * ldc] : 0.0;
+#pragma unroll
+            for (int j = 0; j < 16; ++j) acc[j] = fma(vs[rr][jh + j], c, acc[j]);
+        }
+        __syncthreads();
+    }
+    if (okc)
+#pragma unroll
+        for (int j = 0; j < 16; ++j)
+            if (jh + j < nb) P[((int64_t)blockIdx.y * QR_NB + jh + j) * cols + col] = acc[j];
+}
+
+__global__ __launch_bounds__(256)
+void k_qr_vtc_reduce(int nchunks, int nb, int cols, const double *__restrict__ P,
+                     double *__restrict__ W)
+{
+    const int64_t e = blockIdx.x * (int64_t)256 + threadIdx.x;
+    if (e >= (int64_t)nb * cols) return;
+    const int j = (int)(e / cols), col = (int)(e - (int64_t)j * cols);
+    double s = 0.0;
+    for (int ch = 0; ch < nchunks; ++ch) s += P[((int64_t)ch * QR_NB + j) * cols + col];
+    W[(int64_t)j * cols + col] = s;
+}
+
+// V (rows x nb, ld nb) of the panel's Householder vectors: column j is zero
+// above row j, u_j at row j (already stored in A), A below; columns whose
+// a_j = 0 are zero.
 __global__ __launch_bounds__(256)
 void k_qr_make_v(int rows, int nb, const double *__restrict__ A, int64_t lda,
-                 const double *__restrict__ rd, double *__restrict__ V, double *__restrict__ Vt)
+                 const double *__restrict__ rd, double *__restrict__ V)
 {
     const int64_t e = blockIdx.x * (int64_t)256 + threadIdx.x;
     if (e >= (int64_t)rows * nb) return;
@@ -188,7 +240,6 @@ void k_qr_make_v(int rows, int nb, const double *__restrict__ A, int64_t lda,
     double v = 0.0;
     if (r >= j && rd[j] != 0.0) v = A[(int64_t)r * lda + j];
     V[(int64_t)r * nb + j] = v;
-    Vt[(int64_t)j * rows + r] = v;
 }
 
 // T (nb x nb, upper) of I - V T V^T = H_0 ... H_nb-1 from the Gram matrix
@@ -266,7 +317,7 @@ namespace {
 
 // Workspace of one QR call.
 struct QrWs {
-    double *V, *Vt, *G, *T, *Tt, *W, *W2, *npart, *dpart, *scal;
+    double *V, *G, *T, *Tt, *W, *W2, *npart, *dpart, *scal, *P;
 };
 
 int qr_ws(la_ctx *ctx, int64_t m, int64_t n, int64_t wcols, QrWs &w)
@@ -276,13 +327,13 @@ int qr_ws(la_ctx *ctx, int64_t m, int64_t n, int64_t wcols, QrWs &w)
     const size_t bv = al((size_t)m * QR_NB * 8), bg = al((size_t)QR_NB * QR_NB * 8),
                  bw = al((size_t)QR_NB * std::max<int64_t>(wcols, 1) * 8),
                  bp = al((size_t)QR_PARTS * 8), bd = al((size_t)QR_PARTS * QR_NB * 8),
-                 bs = al((size_t)2 * std::max<int64_t>(dc, 1) * 8);
+                 bs = al((size_t)2 * std::max<int64_t>(dc, 1) * 8),
+                 bpk = al((size_t)VT_MAXCHUNK * QR_NB * std::max<int64_t>(std::max<int64_t>(wcols, 1), QR_NB) * 8);
     void *base = nullptr;
-    int st = ws_get(ctx, WS_QR, 2 * bv + 3 * bg + 2 * bw + bp + bd + bs, &base);
+    int st = ws_get(ctx, WS_QR, bv + 3 * bg + 2 * bw + bp + bd + bs + bpk, &base);
     if (st) return st;
     char *p = (char *)base;
     w.V = (double *)p; p += bv;
-    w.Vt = (double *)p; p += bv;
     w.G = (double *)p; p += bg;
     w.T = (double *)p; p += bg;
     w.Tt = (double *)p; p += bg;
@@ -290,22 +341,37 @@ int qr_ws(la_ctx *ctx, int64_t m, int64_t n, int64_t wcols, QrWs &w)
     w.W2 = (double *)p; p += bw;
     w.npart = (double *)p; p += bp;
     w.dpart = (double *)p; p += bd;
-    w.scal = (double *)p;
+    w.scal = (double *)p; p += bs;
+    w.P = (double *)p;
     return LA_OK;
 }
 
 int grid_rows(int64_t rows) { return (int)std::min<int64_t>(QR_PARTS, std::max<int64_t>(1, ceil_div(rows, 256))); }
 
-// V, Vt and T of the panel at column k0 (width nb) of the packed factor.
+// W (nb x cols, ld cols) = V^T C, split over at most VT_MAXCHUNK row chunks
+int qr_vtc(la_ctx *ctx, int64_t rows, int nb, int64_t cols, const double *C, int64_t ldc,
+           double *W, QrWs &w)
+{
+    const int64_t rch = std::max<int64_t>(VT_SUB, ceil_div(ceil_div(rows, VT_MAXCHUNK), VT_SUB) * VT_SUB);
+    const int nch = (int)ceil_div(rows, rch);
+    hipLaunchKernelGGL(k_qr_vtc_partial, dim3((unsigned)ceil_div(cols, VT_COLS), (unsigned)nch),
+                       dim3(256), 0, ctx->stream, (int)rows, nb, (int)cols, (int)rch,
+                       (const double *)w.V, C, ldc, w.P);
+    hipLaunchKernelGGL(k_qr_vtc_reduce, dim3((unsigned)ceil_div(nb * cols, 256)), dim3(256), 0,
+                       ctx->stream, nch, nb, (int)cols, (const double *)w.P, W);
+    LA_HIP_CHECK(ctx, hipGetLastError());
+    return LA_OK;
+}
+
+// V and T of the panel at column k0 (width nb) of the packed factor.
 int qr_block(la_ctx *ctx, int64_t m, const double *QR, int64_t ldq, const double *rdiag,
              int64_t k0, int nb, QrWs &w)
 {
     const int64_t rows = m - k0;
     hipLaunchKernelGGL(k_qr_make_v, dim3((unsigned)ceil_div(rows * nb, 256)), dim3(256), 0,
-                       ctx->stream, (int)rows, nb, QR + k0 * ldq + k0, ldq, rdiag + k0, w.V, w.Vt);
+                       ctx->stream, (int)rows, nb, QR + k0 * ldq + k0, ldq, rdiag + k0, w.V);
     LA_HIP_CHECK(ctx, hipGetLastError());
-    int st = gemm_rm<double>(ctx, nb, nb, rows, 1.0, w.Vt, rows, w.V, nb, 0.0, w.G, nb, false,
-                             false);
+    int st = qr_vtc(ctx, rows, nb, nb, w.V, nb, w.G, w);
     if (st) return st;
     hipLaunchKernelGGL(k_qr_larft, dim3(1), dim3(64), 0, ctx->stream, nb, (const double *)w.G,
                        rdiag + k0, QR + k0 * ldq + k0, ldq, w.T, w.Tt);
@@ -319,8 +385,7 @@ int qr_apply_block(la_ctx *ctx, int64_t rows, int nb, int64_t cols, double *C, i
                    bool trans, QrWs &w)
 {
     if (cols <= 0) return LA_OK;
-    int st = gemm_rm<double>(ctx, nb, cols, rows, 1.0, w.Vt, rows, C, ldc, 0.0, w.W, cols, false,
-                             false);
+    int st = qr_vtc(ctx, rows, nb, cols, C, ldc, w.W, w);
     if (st) return st;
     st = gemm_rm<double>(ctx, nb, cols, nb, 1.0, trans ? w.Tt : w.T, nb, w.W, cols, 0.0, w.W2,
                          cols, false, false);
