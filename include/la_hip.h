@@ -140,31 +140,31 @@ int la_dgesv_rm(la_ctx *ctx, int64_t n_rows, int64_t n_cols, const double *A,
  * (64 x 64 diagonal blocks inverted, then one GEMM per block). */
 int la_dinverse_rm(la_ctx *ctx, int64_t rows, int64_t cols, const double *A, double *Ainv);
 
-/* Householder QR.  Replaces `QRDecomposition::new` (src/decomp/qr.rs:26-107):
+/* Householder QR.  Replaces `QRDecomposition::new` (src/decomp/qr.rs:26-106):
  * QR (m x n) receives the reference's packed `qr` (R strictly above the
  * diagonal, Householder vector u_k = x - a_k e_k on and below the diagonal
  * of column k, columns with a_k = 0 left as they were) and rdiag
  * (min(m, n)) its `rdiag` (a_k = -sign(x_kk)|x|, +|x| when x_kk <= 0).
  * Blocked: 32-column panels, the panel's reflections applied to the rest as
- * I - V T V^T on MFMA.  get_h, get_r and is_full_rank (qr.rs:109-147) are
+ * I - V T V^T on MFMA.  get_h, get_r and is_full_rank (qr.rs:108-148) are
  * host post-processing of these two outputs; LA_ERR_SHAPE for an empty
  * matrix (mod.rs:209). */
 int la_dgeqrf_rm(la_ctx *ctx, int64_t m, int64_t n, const double *A, double *QR,
                  double *rdiag);
-/* `QRDecomposition::get_q` (qr.rs:149-186): Q (m x m) = the reflections
+/* `QRDecomposition::get_q` (qr.rs:150-194): Q (m x m) = the reflections
  * applied, last first, to the identity restricted to its first min(m, n)
  * diagonal entries. */
 int la_dqr_q_rm(la_ctx *ctx, int64_t m, int64_t n, const double *QR, const double *rdiag,
                 double *Q);
-/* `QRDecomposition::solve` (qr.rs:191-232): X (n x nx) minimising
- * ||Q R X - B||_2.  LA_ERR_SHAPE if b_rows != m (qr.rs:193) or n > m
+/* `QRDecomposition::solve` (qr.rs:196-237): X (n x nx) minimising
+ * ||Q R X - B||_2.  LA_ERR_SHAPE if b_rows != m (qr.rs:200) or n > m
  * (is_full_rank reads rdiag past its end there, a panic in the reference);
- * LA_SINGULAR (-> None) if some rdiag[j] is exactly zero (qr.rs:194-196).
+ * LA_SINGULAR (-> None) if some rdiag[j] is exactly zero (qr.rs:201-203).
  * Two deliberate differences, both defects of the reference: its "Y = Q^T B"
  * step applies x - (<u,x>/u_kk) u, JAMA's formula for vectors normalised to
- * |u|^2 = 2 u_kk, to the unnormalised vectors of qr.rs:75 (|u|^2 = -2 a u_kk),
+ * |u|^2 = 2 u_kk, to the unnormalised vectors of qr.rs:76 (|u|^2 = -2 a u_kk),
  * which is not the reflection; here Q^T B applies the reflections of
- * qr.rs:91-104.  And for m > n the reference builds an n x nx matrix from
+ * qr.rs:92-104.  And for m > n the reference builds an n x nx matrix from
  * m x nx values (a panic); here X is the n x nx least-squares solution. */
 int la_dqrs_rm(la_ctx *ctx, int64_t m, int64_t n, const double *QR, const double *rdiag,
                int64_t b_rows, int64_t nx, const double *B, double *X);

@@ -8,7 +8,7 @@
 //                                 src/matrix/mod.rs:1025-1047
 //   la::LUDecomposition<T>        src/decomp/lu.rs:95-279
 //   la::CholeskyDecomposition<T>  src/decomp/cholesky.rs:51-145
-//   la::QRDecomposition<T>        src/decomp/qr.rs:19-233
+//   la::QRDecomposition<T>        src/decomp/qr.rs:20-237
 //   Matrix::pinverse              src/matrix/mod.rs:1049-1057
 // Reference panics (assert!) throw la::Panic; Option::None is an empty
 // std::optional.  Every f64 factorisation, solve and product runs on the
@@ -278,13 +278,13 @@ private:
     Matrix<T> l_;
 };
 
-// ---- QRDecomposition (qr.rs:19-233) ---------------------------------------
+// ---- QRDecomposition (qr.rs:20-237) ---------------------------------------
 template <typename T>
 class QRDecomposition {
     static_assert(std::is_same<T, double>::value, "the GPU QR path is f64");
 
 public:
-    // QRDecomposition::new (qr.rs:26-44)
+    // QRDecomposition::new (qr.rs:26-43)
     explicit QRDecomposition(const Matrix<T> &a) : qr_(a) {
         const size_t m = a.rows(), n = a.cols();
         std::vector<T> qr(m * n);
@@ -294,7 +294,7 @@ public:
               "QRDecomposition::new");
         qr_ = Matrix<T>(m, n, std::move(qr));
     }
-    // is_full_rank (qr.rs:109-116): rdiag is indexed for every column
+    // is_full_rank (qr.rs:108-115): rdiag is indexed for every column
     bool is_full_rank() const {
         for (size_t j = 0; j < qr_.cols(); ++j) {
             if (j >= rdiag_.size()) throw Panic("is_full_rank: index out of bounds");
@@ -302,7 +302,7 @@ public:
         }
         return true;
     }
-    // get_h (qr.rs:118-132): the lower trapezoid of qr
+    // get_h (qr.rs:117-132): the lower trapezoid of qr
     Matrix<T> get_h() const {
         const size_t m = qr_.rows(), n = qr_.cols();
         std::vector<T> h(m * n);
@@ -310,7 +310,7 @@ public:
             for (size_t j = 0; j < n; ++j) h[i * n + j] = i >= j ? qr_.get_data()[i * n + j] : T(0);
         return Matrix<T>(m, n, std::move(h));
     }
-    // get_r (qr.rs:134-147)
+    // get_r (qr.rs:134-148)
     Matrix<T> get_r() const {
         const size_t m = qr_.rows(), n = qr_.cols();
         std::vector<T> r(m * n);
@@ -319,7 +319,7 @@ public:
                 r[i * n + j] = i < j ? qr_.get_data()[i * n + j] : (i == j ? rdiag_[i] : T(0));
         return Matrix<T>(m, n, std::move(r));
     }
-    // get_q (qr.rs:149-186), m x m
+    // get_q (qr.rs:150-194), m x m
     Matrix<T> get_q() const {
         const size_t m = qr_.rows(), n = qr_.cols();
         std::vector<T> q(m * m);
@@ -328,7 +328,7 @@ public:
               "QRDecomposition::get_q");
         return Matrix<T>(m, m, std::move(q));
     }
-    // solve (qr.rs:191-232) as a least-squares solve: n x nx, None when not
+    // solve (qr.rs:196-237) as a least-squares solve: n x nx, None when not
     // of full rank; the reference's panics (b.rows() != rows, cols > rows)
     // throw Panic.  Where the reference's formula is not the reflection, the
     // reflections are applied (la_hip.h, la_dqrs_rm).

@@ -21,10 +21,10 @@
  *   orc_smul           src/matrix/simd.rs:189-219    (Matrix<f32>::simd_mul)
  *   orc_sdot           src/matrix/simd.rs:166-187    (simd_dot, tail defect
  *                      corrected: the tail adds a[i]*b[i], see below)
- *   orc_qr_new         src/decomp/qr.rs:26-107       (QRDecomposition::new)
- *   orc_qr_is_full_rank qr.rs:109-116
- *   orc_qr_q           src/decomp/qr.rs:149-186      (get_q)
- *   orc_qr_solve_literal qr.rs:191-232 as written; orc_qr_solve the
+ *   orc_qr_new         src/decomp/qr.rs:26-106       (QRDecomposition::new)
+ *   orc_qr_is_full_rank qr.rs:108-115
+ *   orc_qr_q           src/decomp/qr.rs:150-194      (get_q)
+ *   orc_qr_solve_literal qr.rs:196-237 as written; orc_qr_solve the
  *                      least-squares solve it means (see below)
  *   orc_pinverse       src/matrix/mod.rs:1049-1057
  *
@@ -428,7 +428,7 @@ float orc_sdot(int64_t len, const float *a, const float *b)
     return sum;
 }
 
-/* ---- QRDecomposition::new (qr.rs:26-107): unblocked Householder, column by
+/* ---- QRDecomposition::new (qr.rs:26-106): unblocked Householder, column by
  * column.  qr (m*n) receives the packed result: R strictly above the
  * diagonal, the Householder vector u_k = x - a e_k on and below the diagonal
  * of column k (zero columns are left as they were); rdiag (min(m,n))
@@ -439,16 +439,16 @@ int orc_qr_new(int64_t m, int64_t n, const double *a, double *qr, double *rdiag)
     memcpy(qr, a, (size_t)(m * n) * sizeof(double));
     const int64_t diag_count = m < n ? m : n;
     for (int64_t minor = 0; minor < diag_count; ++minor) {
-        double x_norm_sqr = 0.0;                   /* qr.rs:50-54 */
+        double x_norm_sqr = 0.0;                   /* qr.rs:49-53 */
         for (int64_t i = minor; i < m; ++i) {
             double c = qr[i * n + minor];
             x_norm_sqr = x_norm_sqr + c * c;
         }
         double av = qr[minor * n + minor] > 0.0 ? -sqrt(x_norm_sqr) : sqrt(x_norm_sqr);
-        rdiag[minor] = av;                         /* qr.rs:59-60 */
+        rdiag[minor] = av;                         /* qr.rs:58-59 */
         if (av != 0.0) {
-            qr[minor * n + minor] = qr[minor * n + minor] - av;   /* qr.rs:75 */
-            for (int64_t column = minor + 1; column < n; ++column) {   /* qr.rs:91-104 */
+            qr[minor * n + minor] = qr[minor * n + minor] - av;   /* qr.rs:76 */
+            for (int64_t column = minor + 1; column < n; ++column) {   /* qr.rs:92-104 */
                 double x_dot_u = 0.0;
                 for (int64_t row = minor; row < m; ++row) {
                     double p = qr[row * n + minor] * qr[row * n + column];
@@ -466,7 +466,7 @@ int orc_qr_new(int64_t m, int64_t n, const double *a, double *qr, double *rdiag)
     return ORC_OK;
 }
 
-/* ---- QRDecomposition::is_full_rank (qr.rs:109-116): 1 / 0; it indexes
+/* ---- QRDecomposition::is_full_rank (qr.rs:108-115): 1 / 0; it indexes
  * rdiag[j] for every column j, which is out of bounds (a panic) when n > m. */
 int orc_qr_is_full_rank(int64_t m, int64_t n, const double *rdiag)
 {
@@ -477,7 +477,7 @@ int orc_qr_is_full_rank(int64_t m, int64_t n, const double *rdiag)
     return 1;
 }
 
-/* ---- QRDecomposition::get_q (qr.rs:149-186): q (m*m) = H_0 H_1 ... applied
+/* ---- QRDecomposition::get_q (qr.rs:150-194): q (m*m) = H_0 H_1 ... applied
  * to the identity restricted to the first min(m,n) diagonal entries. */
 int orc_qr_q(int64_t m, int64_t n, const double *qr, const double *rdiag, double *q)
 {
@@ -504,7 +504,7 @@ int orc_qr_q(int64_t m, int64_t n, const double *qr, const double *rdiag, double
     return ORC_OK;
 }
 
-/* Shared checks of QRDecomposition::solve (qr.rs:191-232): b.rows() == m
+/* Shared checks of QRDecomposition::solve (qr.rs:196-237): b.rows() == m
  * (assert :193), is_full_rank (:194-196, out-of-bounds panic when n > m). */
 static int orc_qr_solve_checks(int64_t m, int64_t n, const double *rdiag, int64_t b_rows)
 {
@@ -515,10 +515,10 @@ static int orc_qr_solve_checks(int64_t m, int64_t n, const double *rdiag, int64_
     return ORC_OK;
 }
 
-/* ---- QRDecomposition::solve exactly as written (qr.rs:191-232), for m == n
+/* ---- QRDecomposition::solve exactly as written (qr.rs:196-237), for m == n
  * (for m > n the reference's Matrix::new(n, nx, data of m*nx) asserts).  Its
  * "Y = Q^T B" step applies x - (<u,x>/u_kk) u, JAMA's formula for vectors
- * normalised to |u|^2 = 2 u_kk, to the vectors of qr.rs:75, whose norm is
+ * normalised to |u|^2 = 2 u_kk, to the vectors of qr.rs:76, whose norm is
  * |u|^2 = -2 a u_kk: it is not the reflection unless a = -1.  Kept to show
  * that defect; orc_qr_solve is the least-squares solve the routine means. */
 int orc_qr_solve_literal(int64_t m, int64_t n, const double *qr, const double *rdiag,
@@ -553,7 +553,7 @@ int orc_qr_solve_literal(int64_t m, int64_t n, const double *qr, const double *r
 }
 
 /* ---- The least-squares solve of QRDecomposition::solve with the
- * reflections of qr.rs:75-104 (Hx = x + <x,u>/(a u_kk) u, as in new() and
+ * reflections of qr.rs:76-104 (Hx = x + <x,u>/(a u_kk) u, as in new() and
  * get_q), then R x = (Q^T b)[0..n) with R's diagonal rdiag.  m >= n; x is
  * n*nx. */
 int orc_qr_solve(int64_t m, int64_t n, const double *qr, const double *rdiag,
@@ -594,7 +594,7 @@ int orc_qr_solve(int64_t m, int64_t n, const double *qr, const double *rdiag,
 }
 
 /* ---- Matrix::pinverse (mod.rs:1049-1057): (R^T R)^-1 A^T with R = get_r()
- * (qr.rs:134-147: m*n, rdiag on the diagonal, zero below), the products by
+ * (qr.rs:134-148: m*n, rdiag on the diagonal, zero below), the products by
  * impl Mul and the inverse by LU solve with the identity (mod.rs:1034-1037;
  * .unwrap() on None is a panic: ORC_SINGULAR here).  p is n*m. */
 int orc_pinverse(int64_t m, int64_t n, const double *a, double *p)

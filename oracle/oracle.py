@@ -251,7 +251,7 @@ def spd_matrix(n, seed, stream):
 
 
 def qr_new(a):
-    """QRDecomposition::new (src/decomp/qr.rs:26-107) -> (qr, rdiag)."""
+    """QRDecomposition::new (src/decomp/qr.rs:26-106) -> (qr, rdiag)."""
     a = _c(a)
     m, n = a.shape
     qr = np.empty_like(a)
@@ -263,7 +263,7 @@ def qr_new(a):
 
 
 def qr_is_full_rank(qr, rdiag):
-    """QRDecomposition::is_full_rank (qr.rs:109-116); raises on the reference's
+    """QRDecomposition::is_full_rank (qr.rs:108-115); raises on the reference's
     out-of-bounds panic (n > m)."""
     m, n = qr.shape
     r = lib().orc_qr_is_full_rank(m, n, _d(_c(rdiag)))
@@ -273,12 +273,12 @@ def qr_is_full_rank(qr, rdiag):
 
 
 def qr_h(qr):
-    """QRDecomposition::get_h (qr.rs:118-132): the lower trapezoid of qr."""
+    """QRDecomposition::get_h (qr.rs:117-132): the lower trapezoid of qr."""
     return np.tril(qr)
 
 
 def qr_r(qr, rdiag):
-    """QRDecomposition::get_r (qr.rs:134-147): m x n, rdiag on the diagonal."""
+    """QRDecomposition::get_r (qr.rs:134-148): m x n, rdiag on the diagonal."""
     r = np.triu(qr, 1)
     k = len(rdiag)
     r[np.arange(k), np.arange(k)] = rdiag
@@ -286,7 +286,7 @@ def qr_r(qr, rdiag):
 
 
 def qr_q(qr, rdiag):
-    """QRDecomposition::get_q (qr.rs:149-186): m x m."""
+    """QRDecomposition::get_q (qr.rs:150-194): m x m."""
     qr = _c(qr)
     m, n = qr.shape
     q = np.empty((m, m))
@@ -305,8 +305,8 @@ def _qr_solve(fn, qr, rdiag, b):
 
 
 def qr_solve(qr, rdiag, b):
-    """The least-squares solve QRDecomposition::solve means (qr.rs:191-232 with
-    the reflections of qr.rs:75-104): n x nx, or the status on failure."""
+    """The least-squares solve QRDecomposition::solve means (qr.rs:196-237 with
+    the reflections of qr.rs:76-104): n x nx, or the status on failure."""
     st, x = _qr_solve(lib().orc_qr_solve, qr, rdiag, b)
     if st:
         raise OracleError(st)
@@ -314,7 +314,7 @@ def qr_solve(qr, rdiag, b):
 
 
 def qr_solve_literal(qr, rdiag, b):
-    """QRDecomposition::solve exactly as written (qr.rs:191-232; m == n)."""
+    """QRDecomposition::solve exactly as written (qr.rs:196-237; m == n)."""
     st, x = _qr_solve(lib().orc_qr_solve_literal, qr, rdiag, b)
     if st:
         raise OracleError(st)

@@ -1,4 +1,4 @@
-// Householder QR (QRDecomposition, src/decomp/qr.rs:26-232) and the
+// Householder QR (QRDecomposition, src/decomp/qr.rs:26-237) and the
 // pseudo-inverse (Matrix::pinverse, src/matrix/mod.rs:1049-1057).
 //
 // The factor is the reference's: for each column c < min(m, n),
@@ -497,7 +497,7 @@ size_t rbytes(int64_t count, size_t elem)
 {
     return (((size_t)std::max<int64_t>(count, 1) * elem) + 255) & ~(size_t)255;
 }
-// rdiag -> full-rank flag as QRDecomposition::is_full_rank (qr.rs:109-116)
+// rdiag -> full-rank flag as QRDecomposition::is_full_rank (qr.rs:108-115)
 // sees it: LA_ERR_SHAPE when n > m (it reads rdiag past its end, a panic),
 // LA_SINGULAR when some rdiag[j] == 0, else LA_OK.
 int rank_status(la_ctx *ctx, int64_t m, int64_t n, const double *d_rdiag)
@@ -594,12 +594,12 @@ extern "C" int la_dqrs_rm(la_ctx *ctx, int64_t m, int64_t n, const double *QR,
                           double *X)
 {
     if (!ctx) return LA_ERR_ARG;
-    if (b_rows != m) return set_error(ctx, LA_ERR_SHAPE, "qr solve: b.rows() != rows");  // qr.rs:193
+    if (b_rows != m) return set_error(ctx, LA_ERR_SHAPE, "qr solve: b.rows() != rows");  // qr.rs:200
     if (m <= 0 || n <= 0 || nx <= 0) return set_error(ctx, LA_ERR_SHAPE, "qr solve: empty");
     if (!QR || !rdiag || !B || !X) return LA_ERR_ARG;
     if (n > m) return set_error(ctx, LA_ERR_SHAPE, "qr is_full_rank: index out of bounds (cols > rows)");
     for (int64_t j = 0; j < n; ++j)
-        if (rdiag[j] == 0.0) return LA_SINGULAR;                   // qr.rs:194-196: None
+        if (rdiag[j] == 0.0) return LA_SINGULAR;                   // qr.rs:201-203: None
     const int64_t dc = n;
     void *base = nullptr;
     int st = ws_get(ctx, WS_HOST,

@@ -335,7 +335,7 @@ class Context:
         return out
 
     def qr_new(self, a):
-        """QRDecomposition::new (qr.rs:26-107) -> (qr, rdiag)."""
+        """QRDecomposition::new (qr.rs:26-106) -> (qr, rdiag)."""
         a = _c(a)
         m, n = a.shape
         qr = np.empty_like(a)
@@ -344,7 +344,7 @@ class Context:
         return qr, rdiag
 
     def qr_q(self, qr, rdiag):
-        """QRDecomposition::get_q (qr.rs:149-186), m x m."""
+        """QRDecomposition::get_q (qr.rs:150-194), m x m."""
         qr, rdiag = _c(qr), _c(rdiag)
         m, n = qr.shape
         q = np.empty((m, m))
@@ -352,7 +352,7 @@ class Context:
         return q
 
     def qr_solve(self, qr, rdiag, b):
-        """QRDecomposition::solve (qr.rs:191-232) as a least-squares solve:
+        """QRDecomposition::solve (qr.rs:196-237) as a least-squares solve:
         (x or None, status)."""
         qr, rdiag, b = _c(qr), _c(rdiag), _c(b)
         if b.ndim == 1:

@@ -162,27 +162,27 @@ int main() {
         return panics([] { (void)m<D>({{1, 2, 3}, {4, 5, 6}}).is_singular(); });
     });
     run("mod.rs:1567 test_is_non_singular", [] { return m<D>({{2, 6}, {6, 3}}).is_non_singular(); });
-    run("qr.rs:240 qr_test", [] {
+    run("qr.rs:241 qr_test", [] {
         auto a = m<D>({{12, -51, 4}, {6, 167, -68}, {-4, 24, -41}});
         la::QRDecomposition<D> qr(a);
         return approx_eq(qr.get_q() * qr.get_r(), a);
     });
-    run("qr.rs:247 qr_test_m_over_n", [] {
+    run("qr.rs:248 qr_test_m_over_n", [] {
         auto a = m<D>({{1, 2}, {3, 4}, {5, 6}});
         la::QRDecomposition<D> qr(a);
         return approx_eq(qr.get_q() * qr.get_r(), a);
     });
-    run("qr.rs:254 qr_test_n_over_m", [] {
+    run("qr.rs:255 qr_test_n_over_m", [] {
         auto a = m<D>({{1, 2, 3}, {4, 5, 6}});
         la::QRDecomposition<D> qr(a);
         return approx_eq(qr.get_q() * qr.get_r(), a);
     });
-    run("qr.rs:191 solve (least squares, square)", [] {
+    run("qr.rs:196 solve (least squares, square)", [] {
         auto a = m<D>({{12, -51, 4}, {6, 167, -68}, {-4, 24, -41}});
         auto x = la::QRDecomposition<D>(a).solve(m<D>({{1}, {2}, {3}}));
         return x && approx_eq(a * *x, m<D>({{1}, {2}, {3}}));
     });
-    run("qr.rs:109 is_full_rank panics when cols > rows", [] {
+    run("qr.rs:108 is_full_rank panics when cols > rows", [] {
         return panics([] { (void)la::QRDecomposition<D>(m<D>({{1, 2, 3}, {4, 5, 6}})).is_full_rank(); });
     });
     run("mod.rs:1549 test_pinverse", [] {

@@ -1,4 +1,4 @@
-"""GPU tests of the Householder QR (src/decomp/qr.rs:26-232) and of
+"""GPU tests of the Householder QR (src/decomp/qr.rs:26-237) and of
 Matrix::pinverse (src/matrix/mod.rs:1049-1057) through the C ABI.
 
 The GPU factor is the reference's (same sign rule, same Householder vectors,
@@ -21,9 +21,9 @@ from refkit import m, approx_eq
 pytestmark = pytest.mark.gpu
 
 QR_KATS = [
-    m([12.0, -51.0, 4.0], [6.0, 167.0, -68.0], [-4.0, 24.0, -41.0]),   # qr.rs:240-244
-    m([1.0, 2.0], [3.0, 4.0], [5.0, 6.0]),                               # qr.rs:247-251
-    m([1.0, 2.0, 3.0], [4.0, 5.0, 6.0]),                                 # qr.rs:254-258
+    m([12.0, -51.0, 4.0], [6.0, 167.0, -68.0], [-4.0, 24.0, -41.0]),   # qr.rs:241-245
+    m([1.0, 2.0], [3.0, 4.0], [5.0, 6.0]),                               # qr.rs:248-252
+    m([1.0, 2.0, 3.0], [4.0, 5.0, 6.0]),                                 # qr.rs:255-259
 ]
 
 
@@ -33,7 +33,7 @@ def _nrel(x, ref, scale):
 
 @pytest.mark.parametrize("k", range(len(QR_KATS)))
 def test_qr_kats(ctx, k):
-    """qr.rs:240-258: (get_q() * get_r()).approx_eq(a), and the oracle's factor."""
+    """qr.rs:241-259: (get_q() * get_r()).approx_eq(a), and the oracle's factor."""
     a = QR_KATS[k]
     qr, rd = ctx.qr_new(a)
     q = ctx.qr_q(qr, rd)
@@ -68,7 +68,7 @@ def test_qr_vs_oracle(ctx, shape):
 
 def test_qr_zero_column(ctx):
     """a = 0 for a column that is already zero below the diagonal: skipped
-    (qr.rs:64), rdiag 0, the factor not of full rank, the solve None."""
+    (qr.rs:62), rdiag 0, the factor not of full rank, the solve None."""
     a = oracle.normal(5, 1, (70, 40))
     a[:, 35] = 0.0
     qr, rd = ctx.qr_new(a)
@@ -97,7 +97,7 @@ def test_qr_solve(ctx, shape, nx):
 
 
 def test_qr_solve_status_order(ctx):
-    """qr.rs:193 (b rows) before qr.rs:194-196 (full rank); is_full_rank
+    """qr.rs:200 (b rows) before qr.rs:201-203 (full rank); is_full_rank
     reads past rdiag when n > m (a panic: LA_ERR_SHAPE)."""
     a = oracle.normal(1, 1, (5, 5))
     qr, rd = ctx.qr_new(a)

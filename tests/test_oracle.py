@@ -387,9 +387,9 @@ def test_oracle_lu_colcopy_bit_identical(shape):
 
 # --------------------------------------------------------------- QR (qr.rs)
 QR_KATS = [
-    m([12.0, -51.0, 4.0], [6.0, 167.0, -68.0], [-4.0, 24.0, -41.0]),   # qr.rs:240-244
-    m([1.0, 2.0], [3.0, 4.0], [5.0, 6.0]),                               # qr.rs:247-251
-    m([1.0, 2.0, 3.0], [4.0, 5.0, 6.0]),                                 # qr.rs:254-258
+    m([12.0, -51.0, 4.0], [6.0, 167.0, -68.0], [-4.0, 24.0, -41.0]),   # qr.rs:241-245
+    m([1.0, 2.0], [3.0, 4.0], [5.0, 6.0]),                               # qr.rs:248-252
+    m([1.0, 2.0, 3.0], [4.0, 5.0, 6.0]),                                 # qr.rs:255-259
 ]
 
 
@@ -402,7 +402,7 @@ def test_qr_kats(k):
 
 
 def test_qr_against_the_closed_form_of_the_first_kat():
-    """qr.rs:240's matrix has the textbook factor R = [[-14, -21, 14],
+    """qr.rs:241's matrix has the textbook factor R = [[-14, -21, 14],
     [0, -175, 70], [0, 0, 35]] with the reference's sign rule (a = -|x| for
     x_00 = 12 > 0, ...): rdiag and the upper part follow exactly."""
     qr, rd = oracle.qr_new(QR_KATS[0])
@@ -435,8 +435,8 @@ def test_qr_invariants_and_lapack(shape):
 
 
 def test_qr_zero_column_and_rank():
-    """A zero column gives a = 0 and is left as it was (qr.rs:64); the factor
-    is then not of full rank (qr.rs:109-116), and is_full_rank panics (index
+    """A zero column gives a = 0 and is left as it was (qr.rs:62); the factor
+    is then not of full rank (qr.rs:108-115), and is_full_rank panics (index
     out of bounds) when n > m."""
     a = oracle.normal(5, 1, (6, 4))
     a[:, 2] = 0.0
@@ -450,8 +450,8 @@ def test_qr_zero_column_and_rank():
 
 def test_qr_solve_least_squares_and_the_literal_defect():
     """The corrected solve is the least-squares solution (numpy lstsq); the
-    reference's literal formula (qr.rs:203-214, JAMA's normalisation applied
-    to unnormalised vectors) is not: its residual on qr.rs:240's matrix is
+    reference's literal formula (qr.rs:211-222, JAMA's normalisation applied
+    to unnormalised vectors) is not: its residual on qr.rs:241's matrix is
     in the thousands."""
     a = QR_KATS[0]
     b = m([1.0], [2.0], [3.0])
