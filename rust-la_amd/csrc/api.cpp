@@ -79,7 +79,8 @@ ProfScope::~ProfScope() {
 
 static int prof_resolve(la_ctx *ctx) {
     if (ctx->prof_pending.empty()) return LA_OK;
-    LA_HIP_CHECK(ctx, hipEventSynchronize(ctx->prof_pending.back().stop));
+    // kernels may sit on several streams (the LU look-ahead): wait for all
+    for (auto &r : ctx->prof_pending) LA_HIP_CHECK(ctx, hipEventSynchronize(r.stop));
     for (auto &r : ctx->prof_pending) {
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, r.start, r.stop) == hipSuccess) {
@@ -176,6 +177,8 @@ extern "C" void la_ctx_destroy(la_ctx *ctx) {
         hipEventDestroy(r.stop);
     }
     for (auto e : ctx->ev_pool) hipEventDestroy(e);
+    for (auto e : ctx->la_ev)
+        if (e) hipEventDestroy(e);
     for (auto &w : ctx->ws)
         if (w.ptr) hipFree(w.ptr);
     if (ctx->d_scratch) hipFree(ctx->d_scratch);

@@ -15,7 +15,11 @@
 //       row-swap kernel  : net permutation of the panel applied to every
 //                          column outside the panel and to the perm vector
 //       trsm + gemm      : update the rest of the outer block
-//     trsm + gemm        : U12 = L11^-1 A12, A22 -= L21 U12 (MFMA, K = NB_OUT)
+//     u12 + gemm         : U12 = L11^-1 A12 (one kernel per block),
+//                          A22 -= L21 U12 (MFMA, K = NB_OUT)
+// Once the remaining rows fit the CUs reserved for panels (the panel
+// stream's), outer blocks overlap: block j+1's panels run on those CUs while
+// the other CUs apply block j (look-ahead; see getrf_core).
 #include "la_common.hpp"
 #include <mutex>
 
@@ -724,58 +728,56 @@ void k_getrf_panel_local(T *A, int64_t lda, int M, int NB, int kglob, int64_t *i
 }
 
 // ------------------------------------------------------------ row swaps
-// Applies the panel's net row movement to columns [0, c_lo) and [c_hi, n)
-// (the panel's own columns are already final) and to the permutation vector.
-// Each workgroup handles 64 columns: it first gathers every source row of
-// its columns into LDS, then scatters them to their destinations, so cycles
-// in the permutation need no ordering.
+// Applies the net row movement of nmaps consecutive panels (swap maps
+// map_stride words apart, in panel order) to columns [0, c_lo) and
+// [c_hi, n) (the panels' own columns are already final) and to the
+// permutation vector.  Each workgroup handles 64 columns: per map it first
+// gathers every source row of its columns into LDS, then scatters them to
+// their destinations, so cycles in a permutation need no ordering; the maps
+// follow one another behind a barrier.
 template <typename T>
 __global__ __launch_bounds__(256)
 void k_apply_rowmoves(T *A, int64_t lda, int n, int c_lo, int c_hi,
-                      const int64_t *swapmap, uint64_t *perm)
+                      const int64_t *swapmap, int nmaps, int64_t map_stride, uint64_t *perm)
 {
     __shared__ T tile[2 * NB_IN][64];
     __shared__ int64_t s_dst[2 * NB_IN], s_src[2 * NB_IN];
     __shared__ int s_cnt;
     const int tid = threadIdx.x;
-    if (tid == 0) s_cnt = (int)swapmap[0];
-    __syncthreads();
-    const int cnt = s_cnt;
-    if (cnt == 0) return;
-    for (int e = tid; e < cnt; e += blockDim.x) {
-        s_dst[e] = swapmap[1 + 2 * e];
-        s_src[e] = swapmap[2 + 2 * e];
-    }
-    __syncthreads();
-
-    if (blockIdx.x == gridDim.x - 1 && perm) {
-        // permutation vector: perm_new[dst] = perm_old[src]
-        uint64_t v = 0;
-        if (tid < cnt) v = perm[s_src[tid]];
-        __syncthreads();
-        if (tid < cnt) perm[s_dst[tid]] = v;
-        if (cnt > (int)blockDim.x) {
-            // cnt <= 2*NB_IN <= blockDim.x always; kept for clarity
-        }
-        return;
-    }
-    // map this block's 64 columns, skipping the panel columns [c_lo, c_hi)
-    const int left = c_lo;
-    int c0 = blockIdx.x * 64;
-    int col;
-    const int cl = tid & 63;
-    {
-        int v = c0 + cl;
-        col = (v < left) ? v : v + (c_hi - c_lo);
-    }
+    const bool perm_block = blockIdx.x == gridDim.x - 1 && perm;
+    // map this block's 64 columns, skipping the panel columns [c_lo, c_hi);
     // blocks are laid out over (left + right) columns contiguously
-    const bool valid = (c0 + cl) < (left + (n - c_hi));
-    for (int e = tid >> 6; e < cnt; e += blockDim.x >> 6)
-        tile[e][cl] = valid ? A[s_src[e] * lda + col] : T(0);
-    __syncthreads();
-    if (valid)
+    const int cl = tid & 63;
+    const int v = blockIdx.x * 64 + cl;
+    const int col = v < c_lo ? v : v + (c_hi - c_lo);
+    const bool valid = v < c_lo + (n - c_hi);
+    for (int mi = 0; mi < nmaps; ++mi) {
+        const int64_t *map = swapmap + (size_t)mi * map_stride;
+        __syncthreads();                // the previous map's scatter is done
+        if (tid == 0) s_cnt = (int)map[0];
+        __syncthreads();
+        const int cnt = s_cnt;          // <= 2 * NB_IN <= blockDim.x
+        if (cnt == 0) continue;
+        for (int e = tid; e < cnt; e += blockDim.x) {
+            s_dst[e] = map[1 + 2 * e];
+            s_src[e] = map[2 + 2 * e];
+        }
+        __syncthreads();
+        if (perm_block) {
+            // permutation vector: perm_new[dst] = perm_old[src]
+            uint64_t pv = 0;
+            if (tid < cnt) pv = perm[s_src[tid]];
+            __syncthreads();
+            if (tid < cnt) perm[s_dst[tid]] = pv;
+            continue;
+        }
         for (int e = tid >> 6; e < cnt; e += blockDim.x >> 6)
-            A[s_dst[e] * lda + col] = tile[e][cl];
+            tile[e][cl] = valid ? A[s_src[e] * lda + col] : T(0);
+        __syncthreads();
+        if (valid)
+            for (int e = tid >> 6; e < cnt; e += blockDim.x >> 6)
+                A[s_dst[e] * lda + col] = tile[e][cl];
+    }
 }
 
 // ------------------------------------------------------------ unit TRSM
@@ -842,6 +844,97 @@ void k_trsm_unit_lower(const T *Linv, int nb, T *B, int64_t ldb, int ncols)
     }
 }
 
+// ------------------------------------------------------ block U12 = L11^-1 A12
+// U12 for an outer block of nbo <= NB_OUT rows in one pass.  Each workgroup
+// keeps CW columns of A12 (nbo x CW) in LDS and runs the block forward
+// substitution: for each 64-row panel b, X_b <- Linv_b X_b (the panel
+// kernel's inverse, unit diagonal implied; same operation order as
+// k_trsm_unit_lower), then X_r -= L_rb X_b for every panel r below it, with
+// L_rb read from the factored block (Lblk = A + K0 * lda + K0).  Replaces
+// per block four TRSM and three GEMM launches.
+template <typename T, int CW>
+__global__ __launch_bounds__(256)
+void k_u12_block(const T *__restrict__ Lblk, int64_t lda, const T *__restrict__ linv, int nbo,
+                 T *B, int64_t ldb, int ncols)
+{
+    constexpr int RQ = 256 / CW;            // row groups
+    constexpr int KR = 64 / RQ;             // rows per thread per 64-row panel
+    constexpr int XS = CW + 1, LS = 64 + 1;
+    extern __shared__ __align__(16) unsigned char u12_smem[];
+    T *X = reinterpret_cast<T *>(u12_smem);           // NB_OUT x XS
+    T *Ls = X + NB_OUT * XS;                          // 2 x 64 x LS
+    const int tid = threadIdx.x, cl = tid % CW, rq = tid / CW;
+    const int c0 = blockIdx.x * CW;
+    const int col = min(c0 + cl, ncols - 1);
+    const int nb = (nbo + 63) / 64;
+    // stages in order: for each panel b, its inverse (diagonal stage), then
+    // L_rb for r = b+1 .. nb-1; stage s + 1's 64 x 64 operand is loaded into
+    // registers while stage s computes, so the loads overlap the FMAs
+    const int nstage = nb + nb * (nb - 1) / 2;
+    auto stage_br = [&](int st, int &b, int &r) {
+        b = 0;
+        while (st >= nb - b) { st -= nb - b; ++b; }
+        r = b + st;                         // r == b: diagonal stage
+    };
+    T pre[16];
+    auto load_stage = [&](int st) {
+        int b, r;
+        stage_br(st, b, r);
+        const int nbb = min(64, nbo - 64 * b), nbr = min(64, nbo - 64 * r);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int e = tid + 256 * k, i = e >> 6, t = e & 63;
+            pre[k] = r == b ? linv[(size_t)b * NB_IN * NB_IN + min(i, nbb - 1) * NB_IN + min(t, nbb - 1)]
+                            : Lblk[(int64_t)(64 * r + min(i, nbr - 1)) * lda + 64 * b + min(t, nbb - 1)];
+        }
+    };
+    auto store_stage = [&](int st, T *dst) {
+        int b, r;
+        stage_br(st, b, r);
+        const int nbb = min(64, nbo - 64 * b), nbr = min(64, nbo - 64 * r);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int e = tid + 256 * k, i = e >> 6, t = e & 63;
+            const bool in = r == b ? (i < nbb && t < nbb && t != i) : (i < nbr && t < nbb);
+            dst[i * LS + t] = in ? pre[k] : T(0);
+        }
+    };
+    load_stage(0);
+    for (int i = rq; i < NB_OUT; i += RQ) {
+        const T v = B[(int64_t)min(i, nbo - 1) * ldb + col];
+        X[i * XS + cl] = i < nbo ? v : T(0);
+    }
+    store_stage(0, Ls);
+    for (int st = 0; st < nstage; ++st) {
+        T *cur = Ls + (st & 1) * 64 * LS;
+        __syncthreads();                    // stage operand and X rows in place
+        if (st + 1 < nstage) load_stage(st + 1);
+        int b, r;
+        stage_br(st, b, r);
+        T acc[KR];
+#pragma unroll
+        for (int k = 0; k < KR; ++k) acc[k] = r == b ? X[(64 * b + rq + RQ * k) * XS + cl] : T(0);
+#pragma unroll 4
+        for (int t = 0; t < 64; ++t) {
+            const T xb = X[(64 * b + t) * XS + cl];
+#pragma unroll
+            for (int k = 0; k < KR; ++k) acc[k] += cur[(rq + RQ * k) * LS + t] * xb;
+        }
+        if (r == b) {
+            __syncthreads();                // every read of X_b is done
+#pragma unroll
+            for (int k = 0; k < KR; ++k) X[(64 * b + rq + RQ * k) * XS + cl] = acc[k];
+        } else {
+#pragma unroll
+            for (int k = 0; k < KR; ++k) X[(64 * r + rq + RQ * k) * XS + cl] -= acc[k];
+        }
+        if (st + 1 < nstage) store_stage(st + 1, Ls + ((st + 1) & 1) * 64 * LS);
+    }
+    __syncthreads();
+    if (c0 + cl < ncols)
+        for (int i = rq; i < nbo; i += RQ) B[(int64_t)i * ldb + c0 + cl] = X[i * XS + cl];
+}
+
 // sign = parity of the number of real swaps in ipiv (lu.rs:140-152 flips the
 // sign once per p != j).
 __global__ __launch_bounds__(256)
@@ -872,9 +965,64 @@ struct PanelWS {
 constexpr size_t PANEL_LDS_MAX = 150 * 1024;
 
 template <typename T>
+static const void *panel_kernel(int rpl)
+{
+    return rpl == 1 ? (const void *)k_getrf_panel<T, 1>
+         : rpl == 2 ? (const void *)k_getrf_panel<T, 2>
+         : rpl == 3 ? (const void *)k_getrf_panel<T, 3>
+                    : (const void *)k_getrf_panel<T, 4>;
+}
+
+// Rows per lane for a panel of M rows confined to `cus` CUs (an ordinary
+// launch on a CU-masked stream: every workgroup must be resident at once, so
+// the grid may not exceed cus x the kernel's occupancy at the largest LDS
+// block), or 0 when no row count fits.  Aims at `wpc` workgroups per CU
+// (more, shorter workgroups shorten the per-column row work; more than a
+// few per CU contend for the CU), then more rows, then fewer.
+template <typename T>
+static int masked_panel_rpl(int M, int cus)
+{
+    const int rpl_max = sizeof(T) == 8 ? 3 : 4;
+    static int occ[2][5] = {{-1, -1, -1, -1, -1}, {-1, -1, -1, -1, -1}};
+    static std::mutex mu;
+    const int ai = sizeof(T) == 8 ? 0 : 1;
+    auto fits = [&](int rpl) {
+        const int R = 64 * rpl;
+        const int64_t P = ceil_div((int64_t)M, (int64_t)R);
+        int o;
+        {
+            std::lock_guard<std::mutex> g(mu);
+            if (occ[ai][rpl] < 0) {
+                const void *fn = panel_kernel<T>(rpl);
+                const size_t shmem = (size_t)R * (NB_IN + 1) * sizeof(T);
+                int v = 0;
+                if (shmem > allow_dynamic_lds(fn) ||
+                    hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, fn, 256, shmem) != hipSuccess)
+                    v = 0;
+                (void)hipGetLastError();
+                occ[ai][rpl] = v;
+            }
+            o = occ[ai][rpl];
+        }
+        return o > 0 && P <= std::min<int64_t>(PANEL_MAXP, (int64_t)cus * o);
+    };
+    const int wpc = 2;
+    const int r0 = (int)std::min<int64_t>(
+        rpl_max, std::max<int64_t>(1, ceil_div((int64_t)M, (int64_t)64 * cus * wpc)));
+    for (int r = r0; r <= rpl_max; ++r)
+        if (fits(r)) return r;
+    for (int r = r0 - 1; r >= 1; --r)
+        if (fits(r)) return r;
+    return 0;
+}
+
+// mask_cus > 0: the panel runs as an ordinary launch on a stream whose CU
+// mask holds mask_cus CUs (masked_panel_rpl must have accepted M); 0: a
+// cooperative launch over the whole device.
+template <typename T>
 static int launch_panel(la_ctx *ctx, T *Apanel, int64_t lda, int M, int NB, int kglob,
-                        int64_t *d_ipiv, PanelWS &pw, int32_t *d_status, uint64_t ticks,
-                        T *linv)
+                        int64_t *d_ipiv, PanelWS &pw, int64_t *swapmap, int32_t *d_status,
+                        uint64_t ticks, T *linv, int mask_cus = 0)
 {
     // algorithmic traffic: the panel is read and written once
     ProfScope ps(ctx, LA_K_PANEL, (double)M * NB * NB, 2.0 * M * NB * sizeof(T));
@@ -884,7 +1032,7 @@ static int launch_panel(la_ctx *ctx, T *Apanel, int64_t lda, int M, int NB, int 
     const size_t local_bytes = (size_t)M * (NB + 1) * sizeof(T);
     if (local_bytes <= std::min<size_t>(96 * 1024, local_max[ai0])) {
         hipLaunchKernelGGL(k_getrf_panel_local<T>, dim3(1), dim3(256), local_bytes, ctx->stream,
-                           Apanel, lda, M, NB, kglob, d_ipiv, pw.swapmap, linv);
+                           Apanel, lda, M, NB, kglob, d_ipiv, swapmap, linv);
         LA_HIP_CHECK(ctx, hipGetLastError());
         return LA_OK;
     }
@@ -897,10 +1045,12 @@ static int launch_panel(la_ctx *ctx, T *Apanel, int64_t lda, int M, int NB, int 
     const int rpl_max = sizeof(T) == 8 ? 3 : 4;
     const int rpl_pref = sizeof(T) == 8 ? 2 : 1;
     const int rpl_fit = (int)std::max<int64_t>(1, ceil_div(M, (int64_t)64 * maxp));
-    const int rpl = std::min(rpl_max, std::max(rpl_pref, std::min(rpl_fit, rpl_max)));
+    const int rpl = mask_cus > 0 ? masked_panel_rpl<T>(M, mask_cus)
+                                 : std::min(rpl_max, std::max(rpl_pref, std::min(rpl_fit, rpl_max)));
+    if (rpl < 1) return set_error(ctx, LA_ERR_ARG, "getrf: panel does not fit the masked CUs");
     int R = 64 * rpl;
     const int P = (int)ceil_div(M, R);
-    if (P > maxp)
+    if (mask_cus == 0 && P > maxp)
         return set_error(ctx, LA_ERR_ARG,
                          "getrf: more rows than the cooperative panel holds (m > 64 * 3 * "
                          "min(256, CUs) for FP64, 64 * 4 * min(256, CUs) for FP32)");
@@ -918,34 +1068,35 @@ static int launch_panel(la_ctx *ctx, T *Apanel, int64_t lda, int M, int NB, int 
     size_t shmem = (size_t)R * (NB + 1) * sizeof(T);
     static size_t coop_max[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
     const int ai = sizeof(T) == 8 ? 0 : 1;
-    const void *fn = rpl == 1 ? (const void *)k_getrf_panel<T, 1>
-                   : rpl == 2 ? (const void *)k_getrf_panel<T, 2>
-                   : rpl == 3 ? (const void *)k_getrf_panel<T, 3>
-                              : (const void *)k_getrf_panel<T, 4>;
+    const void *fn = panel_kernel<T>(rpl);
     if (!coop_max[ai][rpl - 1]) coop_max[ai][rpl - 1] = allow_dynamic_lds(fn);
     if (shmem > coop_max[ai][rpl - 1])
         return set_error(ctx, LA_ERR_ARG, "getrf panel: rows per workgroup exceed LDS");
     uint64_t *candp = pw.slots;
     uint64_t *diagp = reinterpret_cast<uint64_t *>(pw.diagbuf);
-    int64_t *swapmap = pw.swapmap;
     void *args[] = {&Apanel, &lda, &M, &NB, &R, &kglob, &d_ipiv, &candp, &diagp,
                     &swapmap, &d_status, &ticks, &linv};
-    LA_HIP_CHECK(ctx, hipLaunchCooperativeKernel(fn, dim3(P), dim3(256), args, (unsigned)shmem,
-                                                 ctx->stream));
+    if (mask_cus > 0)
+        LA_HIP_CHECK(ctx, hipLaunchKernel(fn, dim3(P), dim3(256), args, shmem, ctx->stream));
+    else
+        LA_HIP_CHECK(ctx, hipLaunchCooperativeKernel(fn, dim3(P), dim3(256), args,
+                                                     (unsigned)shmem, ctx->stream));
     return LA_OK;
 }
 
 template <typename T>
 static int launch_rowmoves(la_ctx *ctx, T *A, int64_t lda, int64_t n, int64_t c_lo,
-                           int64_t c_hi, const int64_t *swapmap, uint64_t *perm)
+                           int64_t c_hi, const int64_t *swapmap, uint64_t *perm,
+                           int nmaps = 1, int64_t map_stride = 0)
 {
     int64_t ncols = c_lo + (n - c_hi);
+    if (ncols <= 0 && !perm) return LA_OK;
     int nblk = (int)ceil_div(ncols, 64);
-    // up to 2*NB_IN rows move: read + write of each moved row segment
-    ProfScope ps(ctx, LA_K_ROWMOVE, 0.0, 2.0 * 2 * NB_IN * ncols * sizeof(T));
+    // up to 2*NB_IN rows move per map: read + write of each moved row segment
+    ProfScope ps(ctx, LA_K_ROWMOVE, 0.0, 2.0 * 2 * NB_IN * ncols * sizeof(T) * nmaps);
     // one extra block permutes the perm vector
     hipLaunchKernelGGL(k_apply_rowmoves<T>, dim3(nblk + 1), dim3(256), 0, ctx->stream, A,
-                       lda, (int)n, (int)c_lo, (int)c_hi, swapmap, perm);
+                       lda, (int)n, (int)c_lo, (int)c_hi, swapmap, nmaps, map_stride, perm);
     LA_HIP_CHECK(ctx, hipGetLastError());
     return LA_OK;
 }
@@ -958,6 +1109,25 @@ static int launch_trsm(la_ctx *ctx, const T *Linv, int nb, T *B, int64_t ldb, in
                  (double)sizeof(T) * (2.0 * nb * ncols + (double)nb * nb));
     hipLaunchKernelGGL(k_trsm_unit_lower<T>, dim3((unsigned)ceil_div(ncols, 64)), dim3(256), 0,
                        ctx->stream, Linv, nb, B, ldb, (int)ncols);
+    LA_HIP_CHECK(ctx, hipGetLastError());
+    return LA_OK;
+}
+
+template <typename T>
+static int launch_u12_block(la_ctx *ctx, const T *Lblk, int64_t lda, const T *linv, int nbo,
+                            T *B, int64_t ldb, int64_t ncols)
+{
+    constexpr int CW = sizeof(T) == 8 ? 16 : 32;
+    const void *fn = (const void *)k_u12_block<T, CW>;
+    const size_t shmem = ((size_t)NB_OUT * (CW + 1) + 2 * 64 * 65) * sizeof(T);
+    static size_t lds_max[2] = {0, 0};
+    size_t &lm = lds_max[sizeof(T) == 8 ? 0 : 1];
+    if (!lm) lm = allow_dynamic_lds(fn);
+    if (shmem > lm) return set_error(ctx, LA_ERR_ARG, "getrf: U12 block exceeds LDS");
+    ProfScope ps(ctx, LA_K_TRSM, (double)nbo * nbo * ncols,
+                 (double)sizeof(T) * (2.0 * nbo * ncols + (double)nbo * nbo));
+    hipLaunchKernelGGL((k_u12_block<T, CW>), dim3((unsigned)ceil_div(ncols, (int64_t)CW)),
+                       dim3(256), shmem, ctx->stream, Lblk, lda, linv, nbo, B, ldb, (int)ncols);
     LA_HIP_CHECK(ctx, hipGetLastError());
     return LA_OK;
 }
@@ -986,10 +1156,89 @@ constexpr size_t panel_ws_bytes(size_t elem)
            ws_align((size_t)(NB_OUT / NB_IN) * NB_IN * NB_IN * elem);
 }
 
+// ------------------------------------------------ look-ahead streams
+// Two process-wide streams per device with complementary CU masks: the
+// panel stream owns `cus_per_xcd` CUs of every XCD, the update stream the
+// rest.  On MI355X mask bit i selects a CU of XCD i % 8 and, within it, of
+// shader engine (i / 8) % 4, so bits [0, 8 * cus_per_xcd) spread the panel
+// evenly over XCDs and, when cus_per_xcd is a multiple of 4, over the four
+// shader engines of each XCD.  That evenness is required, not cosmetic: an
+// XCD deals workgroups to its shader engines in turn, so with an uneven
+// split (say 3, 3, 2, 2 CUs) the smaller engines overflow and part of the
+// grid waits for CUs the rest of it never frees.  The runtime also refuses
+// a mask that leaves an XCD with no CU.
+// Cooperative launches ignore CU masks, which is why the masked panel is an
+// ordinary launch sized by masked_panel_rpl.  The streams live until the
+// process ends (the runtime allows only a few masked hardware queues).
+struct LookAhead {
+    hipStream_t panel = nullptr, update = nullptr;
+    int panel_cus = 0;
+    bool tried = false;
+};
+
+static const LookAhead *lookahead_streams(la_ctx *ctx)
+{
+    static std::mutex mu;
+    static LookAhead la[16];
+    const char *off = getenv("LAHIP_LOOKAHEAD");
+    if (off && atoi(off) == 0) return nullptr;
+    if (ctx->device < 0 || ctx->device >= 16 || ctx->num_cus < 16 || ctx->num_cus % 8 != 0)
+        return nullptr;
+    std::lock_guard<std::mutex> g(mu);
+    LookAhead &l = la[ctx->device];
+    if (!l.tried) {
+        l.tried = true;
+        int per_xcd = 12;
+        if (const char *e = getenv("LAHIP_PANEL_CUS_PER_XCD")) per_xcd = atoi(e);
+        per_xcd -= per_xcd % 4;
+        const int xcd_cus = ctx->num_cus / 8;
+        if (per_xcd < 4 || per_xcd >= xcd_cus || xcd_cus % 4 != 0) return nullptr;
+        const int words = (ctx->num_cus + 31) / 32;
+        std::vector<uint32_t> mp(words, 0), mu_(words, 0);
+        for (int i = 0; i < ctx->num_cus; ++i)
+            (i < 8 * per_xcd ? mp : mu_)[i / 32] |= 1u << (i % 32);
+        int cur = -1;
+        if (hipGetDevice(&cur) != hipSuccess) return nullptr;
+        if (cur != ctx->device && hipSetDevice(ctx->device) != hipSuccess) return nullptr;
+        hipStream_t sp = nullptr, su = nullptr;
+        bool ok = hipExtStreamCreateWithCUMask(&sp, (uint32_t)words, mp.data()) == hipSuccess &&
+                  hipExtStreamCreateWithCUMask(&su, (uint32_t)words, mu_.data()) == hipSuccess;
+        (void)hipGetLastError();
+        if (cur != ctx->device) hipSetDevice(cur);
+        if (!ok) return nullptr;
+        l.panel = sp;
+        l.update = su;
+        l.panel_cus = 8 * per_xcd;
+    }
+    return l.panel ? &l : nullptr;
+}
+
+// Restores ctx->stream when the look-ahead schedule returns.
+struct StreamSwap {
+    la_ctx *ctx;
+    hipStream_t saved;
+    explicit StreamSwap(la_ctx *c) : ctx(c), saved(c->stream) {}
+    void use(hipStream_t s) { ctx->stream = s; }
+    ~StreamSwap() { ctx->stream = saved; }
+};
+
 // In-place LU.  d_ipiv receives LAPACK-style swaps (global rows), d_perm the
 // permutation vector (may be null), d_status LA_OK / LA_ERR_TIMEOUT.
 // ext_ws: caller-provided panel workspace of panel_ws_bytes(sizeof(T))
 // bytes, or null for the context's grow-only workspace.
+//
+// Outer blocks run one of two schedules.  The plain schedule factors a block
+// and then updates everything to its right, all on ctx->stream.  Once the
+// remaining rows fit the panel stream's CUs, blocks run with look-ahead: the
+// panel stream factors block j+1 (panels, their row moves inside the block,
+// the in-block updates) while the update stream applies block j to the
+// columns right of block j+1.  Per block j the update stream does, in order:
+// block j's row moves, U12 = L11^-1 A12 and update on block j+1's columns
+// (after which block j+1 may start), then block j's row moves on the columns
+// left of it and right of block j+1 (and the permutation vector), U12 and
+// the update there.  Every location is written by one stream
+// between two event waits, and row moves on a column keep their block order,
+// so the factor is identical to the plain schedule's.
 template <typename T>
 int getrf_core(la_ctx *ctx, int64_t m, int64_t n, T *A, int64_t lda, int64_t *d_ipiv,
                uint64_t *d_perm, int32_t *d_status, int upd_prec, void *ext_ws)
@@ -997,15 +1246,24 @@ int getrf_core(la_ctx *ctx, int64_t m, int64_t n, T *A, int64_t lda, int64_t *d_
     if (m > INT32_MAX / 2 || n > INT32_MAX / 2)
         return set_error(ctx, LA_ERR_ARG, "getrf: extent too large");
     const int64_t kmax = m < n ? m : n;
+    // outer block from which the look-ahead schedule runs (kmax: never)
+    const LookAhead *la = ext_ws ? nullptr : lookahead_streams(ctx);
+    int64_t K0la = kmax;
+    if (la)
+        for (int64_t K0 = 0; K0 + NB_OUT < kmax && K0 + NB_OUT < n; K0 += NB_OUT)
+            if (masked_panel_rpl<T>((int)(m - K0), la->panel_cus) > 0) { K0la = K0; break; }
     // panel workspace
     size_t slot_bytes = (size_t)2 * PANEL_MAXP * CAND_WORDS * sizeof(uint64_t);
     size_t row_bytes = 256;
     size_t diag_bytes = (size_t)2 * 2 * NB_IN * sizeof(uint64_t);
     size_t map_bytes = (size_t)(1 + 4 * NB_IN) * sizeof(int64_t);
+    constexpr int RING = 2 * (NB_OUT / NB_IN);           // swap maps of two blocks
     auto al = [](size_t x) { return ws_align(x); };
     void *base = ext_ws;
     int st;
-    if (!base && (st = ws_get(ctx, WS_PANEL, panel_ws_bytes(sizeof(T)), &base))) return st;
+    const size_t ring_bytes = K0la < kmax ? RING * al(map_bytes) : 0;
+    if (!base && (st = ws_get(ctx, WS_PANEL, panel_ws_bytes(sizeof(T)) + ring_bytes, &base)))
+        return st;
     PanelWS pw;
     char *b = (char *)base;
     pw.slots = (uint64_t *)b; b += al(slot_bytes);
@@ -1013,8 +1271,10 @@ int getrf_core(la_ctx *ctx, int64_t m, int64_t n, T *A, int64_t lda, int64_t *d_
     pw.diagbuf = b; b += al(diag_bytes);
     pw.swapmap = (int64_t *)b; b += al(map_bytes);
     T *linv_ring = (T *)b;
+    b += al((size_t)(NB_OUT / NB_IN) * NB_IN * NB_IN * sizeof(T));
+    int64_t *swap_ring = (int64_t *)b;
+    const size_t ring_stride = al(map_bytes) / sizeof(int64_t);
 
-    // reset slots: step tag 0xffffffff never occurs
     // step tag 0xffffffff never occurs: no stale word can validate
     const int64_t nslot_words = 2 * PANEL_MAXP * CAND_WORDS;
     if ((st = fill_u64(ctx, pw.slots, nslot_words, 0xffffffff00000000ull))) return st;
@@ -1024,51 +1284,125 @@ int getrf_core(la_ctx *ctx, int64_t m, int64_t n, T *A, int64_t lda, int64_t *d_
     if (d_perm && (st = iota_u64(ctx, d_perm, m))) return st;
     const uint64_t ticks = spin_limit_ticks(ctx);
 
-    for (int64_t K0 = 0; K0 < kmax; K0 += NB_OUT) {
-        const int64_t nbo = std::min<int64_t>(NB_OUT, kmax - K0);
+    // Factors the nbo columns of block K0 (panels, their row moves on the
+    // block's other columns, in-block updates).  outside_moves: also apply
+    // each panel's moves to the columns outside the block and to perm.
+    auto factor_block = [&](int64_t K0, int64_t nbo, int64_t *maps, size_t map_stride,
+                            int mask_cus, bool outside_moves) -> int {
         for (int64_t k = K0; k < K0 + nbo; k += NB_IN) {
             const int nbi = (int)std::min<int64_t>(NB_IN, K0 + nbo - k);
             T *Ap = A + k * lda + k;
             T *linv = linv_ring + (size_t)((k - K0) / NB_IN) * NB_IN * NB_IN;
-            if ((st = launch_panel<T>(ctx, Ap, lda, (int)(m - k), nbi, (int)k, d_ipiv, pw,
-                                      d_status, ticks, linv)))
-                return st;
-            if ((st = launch_rowmoves<T>(ctx, A, lda, n, k, k + nbi, pw.swapmap, d_perm)))
-                return st;
+            int64_t *smap = maps + (size_t)((k - K0) / NB_IN) * map_stride;
+            int s;
+            if ((s = launch_panel<T>(ctx, Ap, lda, (int)(m - k), nbi, (int)k, d_ipiv, pw, smap,
+                                     d_status, ticks, linv, mask_cus)))
+                return s;
+            if (outside_moves) {
+                if ((s = launch_rowmoves<T>(ctx, A, lda, n, k, k + nbi, smap, d_perm))) return s;
+            } else if (nbo > nbi) {
+                if ((s = launch_rowmoves<T>(ctx, A + K0, lda, nbo, k - K0, k - K0 + nbi, smap,
+                                            nullptr)))
+                    return s;
+            }
             const int64_t rest = K0 + nbo - (k + nbi);
             if (rest > 0) {
-                if ((st = launch_trsm<T>(ctx, linv, nbi, A + k * lda + k + nbi, lda, rest)))
-                    return st;
+                if ((s = launch_trsm<T>(ctx, linv, nbi, A + k * lda + k + nbi, lda, rest)))
+                    return s;
                 if (m - k - nbi > 0)
-                    if ((st = trail_gemm<T>(ctx, upd_prec, m - k - nbi, rest, nbi,
-                                            A + (k + nbi) * lda + k, lda, A + k * lda + k + nbi,
-                                            lda, A + (k + nbi) * lda + k + nbi, lda)))
-                        return st;
+                    if ((s = trail_gemm<T>(ctx, upd_prec, m - k - nbi, rest, nbi,
+                                           A + (k + nbi) * lda + k, lda, A + k * lda + k + nbi,
+                                           lda, A + (k + nbi) * lda + k + nbi, lda)))
+                        return s;
             }
         }
-        const int64_t ncol = n - (K0 + nbo);
-        if (ncol > 0) {
-            T *B12 = A + K0 * lda + K0 + nbo;
-            for (int64_t kb = K0; kb < K0 + nbo; kb += NB_IN) {
-                const int nbi = (int)std::min<int64_t>(NB_IN, K0 + nbo - kb);
-                const T *linv_kb = linv_ring + (size_t)((kb - K0) / NB_IN) * NB_IN * NB_IN;
-                if ((st = launch_trsm<T>(ctx, linv_kb, nbi, A + kb * lda + K0 + nbo, lda, ncol)))
-                    return st;
-                const int64_t below = K0 + nbo - (kb + nbi);
-                if (below > 0)
-                    if ((st = trail_gemm<T>(ctx, upd_prec, below, ncol, nbi,
-                                            A + (kb + nbi) * lda + kb, lda,
-                                            A + kb * lda + K0 + nbo, lda,
-                                            A + (kb + nbi) * lda + K0 + nbo, lda)))
-                        return st;
-            }
-            if (K0 + nbo < m)
-                if ((st = trail_gemm<T>(ctx, upd_prec, m - K0 - nbo, ncol, nbo,
-                                        A + (K0 + nbo) * lda + K0, lda, B12, lda,
-                                        A + (K0 + nbo) * lda + K0 + nbo, lda)))
-                    return st;
+        return LA_OK;
+    };
+    // U12 = L11^-1 A12 for block K0 on columns [c0, c0 + ncol) (the block's rows).
+    auto block_u12 = [&](int64_t K0, int64_t nbo, int64_t c0, int64_t ncol) -> int {
+        if (ncol <= 0) return LA_OK;
+        if (upd_prec != LA_PREC_BF16 && upd_prec != LA_PREC_FP16)
+            return launch_u12_block<T>(ctx, A + K0 * lda + K0, lda, linv_ring, (int)nbo,
+                                       A + K0 * lda + c0, lda, ncol);
+        for (int64_t kb = K0; kb < K0 + nbo; kb += NB_IN) {
+            const int nbi = (int)std::min<int64_t>(NB_IN, K0 + nbo - kb);
+            const T *linv_kb = linv_ring + (size_t)((kb - K0) / NB_IN) * NB_IN * NB_IN;
+            int s;
+            if ((s = launch_trsm<T>(ctx, linv_kb, nbi, A + kb * lda + c0, lda, ncol))) return s;
+            const int64_t below = K0 + nbo - (kb + nbi);
+            if (below > 0)
+                if ((s = trail_gemm<T>(ctx, upd_prec, below, ncol, nbi, A + (kb + nbi) * lda + kb,
+                                       lda, A + kb * lda + c0, lda, A + (kb + nbi) * lda + c0,
+                                       lda)))
+                    return s;
+        }
+        return LA_OK;
+    };
+    // A22 -= L21 U12 for block K0 on columns [c0, c0 + ncol).
+    auto block_gemm = [&](int64_t K0, int64_t nbo, int64_t c0, int64_t ncol) -> int {
+        if (ncol <= 0 || K0 + nbo >= m) return LA_OK;
+        return trail_gemm<T>(ctx, upd_prec, m - K0 - nbo, ncol, nbo, A + (K0 + nbo) * lda + K0,
+                             lda, A + K0 * lda + c0, lda, A + (K0 + nbo) * lda + c0, lda);
+    };
+
+    // plain schedule
+    for (int64_t K0 = 0; K0 < K0la; K0 += NB_OUT) {
+        const int64_t nbo = std::min<int64_t>(NB_OUT, kmax - K0);
+        if ((st = factor_block(K0, nbo, pw.swapmap, 0, 0, true))) return st;
+        if (n - (K0 + nbo) > 0) {
+            if ((st = block_u12(K0, nbo, K0 + nbo, n - (K0 + nbo)))) return st;
+            if ((st = block_gemm(K0, nbo, K0 + nbo, n - (K0 + nbo)))) return st;
         }
     }
+    if (K0la >= kmax) return LA_OK;
+
+    // look-ahead schedule
+    if (!ctx->la_ev[0])
+        for (auto &e : ctx->la_ev)
+            LA_HIP_CHECK(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    hipEvent_t ev_start = ctx->la_ev[0], ev_pan = ctx->la_ev[1], ev_la = ctx->la_ev[2],
+               ev_end = ctx->la_ev[3];
+    const hipStream_t home = ctx->stream;
+    StreamSwap sw(ctx);
+    LA_HIP_CHECK(ctx, hipEventRecord(ev_start, home));
+    LA_HIP_CHECK(ctx, hipStreamWaitEvent(la->panel, ev_start, 0));
+    LA_HIP_CHECK(ctx, hipStreamWaitEvent(la->update, ev_start, 0));
+    int parity = 0;
+    for (int64_t K0 = K0la; K0 < kmax; K0 += NB_OUT, parity ^= 1) {
+        const int64_t nbo = std::min<int64_t>(NB_OUT, kmax - K0);
+        int64_t *maps = swap_ring + (size_t)parity * (NB_OUT / NB_IN) * ring_stride;
+        const int npan = (int)ceil_div(nbo, (int64_t)NB_IN);
+        sw.use(la->panel);
+        if (K0 > K0la) LA_HIP_CHECK(ctx, hipStreamWaitEvent(la->panel, ev_la, 0));
+        if ((st = factor_block(K0, nbo, maps, ring_stride, la->panel_cus, false))) return st;
+        LA_HIP_CHECK(ctx, hipEventRecord(ev_pan, la->panel));
+
+        // update stream: first what block j+1 needs (block j's row moves,
+        // U12 and update on block j+1's columns [c1, c1 + nla)), then block
+        // j's moves on the columns left of it and right of block j+1 (one
+        // launch, with the permutation vector), U12 and the update there
+        sw.use(la->update);
+        LA_HIP_CHECK(ctx, hipStreamWaitEvent(la->update, ev_pan, 0));
+        const int64_t c1 = K0 + nbo, ncol = n - c1;
+        const int64_t nla = std::min<int64_t>(NB_OUT, std::max<int64_t>(ncol, 0));
+        if (nla > 0) {
+            if ((st = launch_rowmoves<T>(ctx, A + c1, lda, nla, 0, 0, maps, nullptr, npan,
+                                         (int64_t)ring_stride)))
+                return st;
+            if ((st = block_u12(K0, nbo, c1, nla))) return st;
+            if ((st = block_gemm(K0, nbo, c1, nla))) return st;
+        }
+        LA_HIP_CHECK(ctx, hipEventRecord(ev_la, la->update));
+        if ((st = launch_rowmoves<T>(ctx, A, lda, n, K0, c1 + nla, maps, d_perm, npan,
+                                     (int64_t)ring_stride)))
+            return st;
+        if ((st = block_u12(K0, nbo, c1 + nla, ncol - nla))) return st;
+        if ((st = block_gemm(K0, nbo, c1 + nla, ncol - nla))) return st;
+    }
+    LA_HIP_CHECK(ctx, hipEventRecord(ev_pan, la->panel));
+    LA_HIP_CHECK(ctx, hipEventRecord(ev_end, la->update));
+    LA_HIP_CHECK(ctx, hipStreamWaitEvent(home, ev_pan, 0));
+    LA_HIP_CHECK(ctx, hipStreamWaitEvent(home, ev_end, 0));
     return LA_OK;
 }
 
