@@ -203,7 +203,12 @@ int la_dgemm_dev(la_ctx *ctx, int64_t m, int64_t n, int64_t k,
  * factored by at most min(256, CUs) co-resident workgroups of at most 192
  * rows each (FP64; 256 for the FP32 factor of la_dgesv_ir_dev), so
  * m <= 49152 on a 256-CU MI355X (65536 for the FP32 factor); beyond it the
- * call returns LA_ERR_ARG before launching anything. */
+ * call returns LA_ERR_ARG before launching anything.  Outer blocks whose rows
+ * fit 96 CUs run with look-ahead on two process-wide streams with
+ * complementary CU masks (panels of block j+1 beside the update of block j),
+ * ordered after earlier work on ctx's stream and before later work on it by
+ * events, so the call stays stream-ordered; the factor is bit-identical to
+ * the single-stream schedule's, which LAHIP_LOOKAHEAD=0 selects. */
 int la_dgetrf_dev(la_ctx *ctx, int64_t m, int64_t n, double *A, int64_t lda,
                   uint64_t *d_piv, int32_t *d_info);
 
@@ -217,7 +222,8 @@ int la_dgetrf_dev(la_ctx *ctx, int64_t m, int64_t n, double *A, int64_t lda,
  * beyond).  The scratch is shared by every call on one device and keeps the
  * last call's contents (each call resets what it reads first): calls that
  * overlap on one device, from any context or stream, race.  The timing
- * harness is suspended for the call. */
+ * harness is suspended for the call.  It always runs the single-stream
+ * schedule (no look-ahead), whose factor is bit-identical. */
 int la_dgetrf_noalloc_dev(la_ctx *ctx, int64_t m, int64_t n, double *A, int64_t lda,
                           uint64_t *d_piv, int32_t *d_info);
 

@@ -1361,12 +1361,29 @@ int getrf_core(la_ctx *ctx, int64_t m, int64_t n, T *A, int64_t lda, int64_t *d_
         for (auto &e : ctx->la_ev)
             LA_HIP_CHECK(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
     hipEvent_t ev_start = ctx->la_ev[0], ev_pan = ctx->la_ev[1], ev_la = ctx->la_ev[2],
-               ev_end = ctx->la_ev[3];
+               ev_end = ctx->la_ev[3], ev_u12 = ctx->la_ev[4], ev_share = ctx->la_ev[5];
     const hipStream_t home = ctx->stream;
     StreamSwap sw(ctx);
     LA_HIP_CHECK(ctx, hipEventRecord(ev_start, home));
     LA_HIP_CHECK(ctx, hipStreamWaitEvent(la->panel, ev_start, 0));
     LA_HIP_CHECK(ctx, hipStreamWaitEvent(la->update, ev_start, 0));
+    // Share of the update handed to the panel stream.  While the updates are
+    // large the panel stream finishes block j+1 long before the update
+    // stream finishes block j's update, so it takes the rightmost columns of
+    // that update once block j+1 is factored.  The share is sized from a
+    // rate model (GEMM FLOP/s per CU at K = 256 and panel time per column,
+    // both measured on MI355X at n = 16384) so that both streams finish
+    // together (scaling the modelled share by 0.5 or 1.5 was slower); it never includes the columns of
+    // blocks j+1 and j+2, which the update stream needs first.  Fencing: the
+    // panel stream's share waits for block j's U12 there (ev_u12); the update
+    // stream waits for the share (ev_share) before block j+1's row moves,
+    // which rewrite rows of block j's L21 that the share reads.
+    const double gemm_tf_per_cu = sizeof(T) == 8 ? 0.20e12 : 0.354e12;
+    const double panel_s_per_col = sizeof(T) == 8 ? 4.9e-6 : 4.3e-6;
+    const double r_upd = gemm_tf_per_cu * (ctx->num_cus - la->panel_cus);
+    const double r_pan = gemm_tf_per_cu * la->panel_cus;
+    struct Share { int64_t K0 = 0, nbo = 0, c0 = 0, ncol = 0; } share;   // pending (block j-1)
+    bool share_pending = false;
     int parity = 0;
     for (int64_t K0 = K0la; K0 < kmax; K0 += NB_OUT, parity ^= 1) {
         const int64_t nbo = std::min<int64_t>(NB_OUT, kmax - K0);
@@ -1376,6 +1393,13 @@ int getrf_core(la_ctx *ctx, int64_t m, int64_t n, T *A, int64_t lda, int64_t *d_
         if (K0 > K0la) LA_HIP_CHECK(ctx, hipStreamWaitEvent(la->panel, ev_la, 0));
         if ((st = factor_block(K0, nbo, maps, ring_stride, la->panel_cus, false))) return st;
         LA_HIP_CHECK(ctx, hipEventRecord(ev_pan, la->panel));
+        const bool shared_prev = share_pending;
+        if (share_pending) {
+            LA_HIP_CHECK(ctx, hipStreamWaitEvent(la->panel, ev_u12, 0));
+            if ((st = block_gemm(share.K0, share.nbo, share.c0, share.ncol))) return st;
+            LA_HIP_CHECK(ctx, hipEventRecord(ev_share, la->panel));
+            share_pending = false;
+        }
 
         // update stream: first what block j+1 needs (block j's row moves,
         // U12 and update on block j+1's columns [c1, c1 + nla)), then block
@@ -1393,11 +1417,32 @@ int getrf_core(la_ctx *ctx, int64_t m, int64_t n, T *A, int64_t lda, int64_t *d_
             if ((st = block_gemm(K0, nbo, c1, nla))) return st;
         }
         LA_HIP_CHECK(ctx, hipEventRecord(ev_la, la->update));
+        if (shared_prev) LA_HIP_CHECK(ctx, hipStreamWaitEvent(la->update, ev_share, 0));
         if ((st = launch_rowmoves<T>(ctx, A, lda, n, K0, c1 + nla, maps, d_perm, npan,
                                      (int64_t)ring_stride)))
             return st;
         if ((st = block_u12(K0, nbo, c1 + nla, ncol - nla))) return st;
-        if ((st = block_gemm(K0, nbo, c1 + nla, ncol - nla))) return st;
+        const int64_t nbulk = ncol - nla, rows = m - c1;
+        int64_t nshare = 0;
+        if (nbulk > 2 * NB_OUT && rows > 0 && K0 + 2 * NB_OUT < kmax) {
+            const double w = 2.0 * rows * nbulk * nbo;
+            const double t_next = panel_s_per_col * std::min<int64_t>(NB_OUT, kmax - c1);
+            const double ws = (w / r_upd - t_next) / (1.0 / r_pan + 1.0 / r_upd);
+            if (ws > 0) nshare = (int64_t)(ws / (2.0 * rows * nbo)) / 128 * 128;
+            nshare = std::min<int64_t>(nshare, nbulk - NB_OUT);
+            if (nshare < 256) nshare = 0;
+        }
+        if (nshare > 0) {
+            LA_HIP_CHECK(ctx, hipEventRecord(ev_u12, la->update));
+            share = Share{K0, nbo, n - nshare, nshare};
+            share_pending = true;
+        }
+        if ((st = block_gemm(K0, nbo, c1 + nla, nbulk - nshare))) return st;
+    }
+    if (share_pending) {            // not reached: the last two blocks never share
+        sw.use(la->panel);
+        LA_HIP_CHECK(ctx, hipStreamWaitEvent(la->panel, ev_u12, 0));
+        if ((st = block_gemm(share.K0, share.nbo, share.c0, share.ncol))) return st;
     }
     LA_HIP_CHECK(ctx, hipEventRecord(ev_pan, la->panel));
     LA_HIP_CHECK(ctx, hipEventRecord(ev_end, la->update));

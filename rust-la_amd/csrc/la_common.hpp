@@ -86,7 +86,7 @@ struct la_ctx {
     bool lu_tuned = false;
     bool prof = false;
     std::vector<hipEvent_t> ev_pool;
-    hipEvent_t la_ev[4] = {nullptr, nullptr, nullptr, nullptr};   // LU look-ahead joins
+    hipEvent_t la_ev[6] = {};       // LU look-ahead stream joins
     std::vector<lahip::ProfRecord> prof_pending;
     lahip::ProfStats prof_stats[LA_K_COUNT];
 };
