@@ -6,10 +6,14 @@
 //   u = x - a_c e_c stored in place (A[c][c] = x_cc - a_c),  rdiag[c] = a_c,
 //   every later column y -> y + <y,u>/(a_c u_c) u  (the reflection H_c),
 // skipped when a_c = 0.  It is computed blocked, 32 columns per panel:
-//   * inside the panel, column by column with two small launches per column
-//     (the dot products of u with the other panel columns, then their
-//     update fused with the next column's partial norm), every reduction in
-//     a fixed order so that the result is the same in every run;
+//   * inside the panel, column by column in one cooperative kernel
+//     (k_qr_panel) with one exchange between workgroups per column: the
+//     norm of x and the products <x, y> for the panel's other columns are
+//     summed in one pass, and <u, y> = <x, y> - a_c y_c then needs only the
+//     diagonal row, which travels in the same exchange; every sum is taken
+//     in a fixed order, so the result is the same in every run and in every
+//     workgroup.  Panels of more than 65536 rows use two small launches per
+//     column instead (k_qr_dots, k_qr_apply);
 //   * the panel's reflections as one block, H_c H_c+1 ... = I - V T V^T
 //     (T upper triangular, T_jj = tau_j = -1/(a_j u_j), forward column-wise
 //     accumulation of Schreiber and Van Loan), applied to the trailing
@@ -27,6 +31,172 @@ namespace lahip {
 
 constexpr int QR_NB = 32;       // panel width
 constexpr int QR_PARTS = 256;   // max workgroups of the column kernels
+constexpr int QRP_MAXP = 128;   // max workgroups of the cooperative panel
+constexpr int QRP_SW = 4 * QR_NB;   // slot words: 32 partial sums, then (workgroup 0) the diagonal row
+constexpr int QRP_LS = QR_NB + 1;   // LDS row stride of the panel rows (doubles)
+
+__device__ inline void put_tagged(uint64_t *w, uint32_t tag, double v)
+{
+    const uint64_t b = (uint64_t)__double_as_longlong(v), t = (uint64_t)tag << 32;
+    st_dev(w + 0, t | (b & 0xffffffffull));
+    st_dev(w + 1, t | (b >> 32));
+}
+
+// Waits (bounded) until the 64-bit word at p carries `tag` in its high half.
+// Returns false when the wait is abandoned: the sticky status is set (by
+// this or another workgroup) or the spin limit passes.
+__device__ inline bool wait_tagged(const uint64_t *p, uint32_t tag, uint64_t &v,
+                                   int32_t *status, uint64_t spin_ticks)
+{
+    if ((uint32_t)(v >> 32) == tag) return true;
+    const uint64_t t0 = wall_clock64();
+    for (uint32_t spin = 0;; ++spin) {
+        v = ld_dev(p);
+        if ((uint32_t)(v >> 32) == tag) return true;
+        if ((spin & 31) == 31 && __hip_atomic_load(status, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT) != 0)
+            return false;
+        if (wall_clock64() - t0 > spin_ticks) {
+            atomicCAS((int *)status, 0, (int)LA_ERR_TIMEOUT);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+// The panel of columns [k0, k0 + nb) over rows [k0, m) in one cooperative
+// launch of P workgroups; workgroup w keeps rows [k0 + w R, k0 + (w+1) R),
+// R = 256 RPT, in LDS.  Per column c (global gc = k0 + c, whose diagonal row
+// is row c of workgroup 0):
+//   1. partials over the workgroup's rows r >= gc: q_j = sum_r x_r A[r][j]
+//      for j in [c, nb) (j = c is |x|^2), thread (g, j) over row group g,
+//      then the eight groups in order;
+//   2. publish q (and, workgroup 0, the diagonal row A[gc][c..nb)) as tagged
+//      words, buffers alternating by column parity;
+//   3. every workgroup reads all P partials (thread (g, j) the workgroups
+//      g, g + 8, ...) and sums them in one fixed order;
+//   4. a = -sign(x_c)|x|, u_c = x_c - a, f_j = (q_j - a A[gc][j]) / (a u_c),
+//      and its rows r >= gc get A[r][j] += f_j u_r (u_gc = u_c), A[gc][c] = u_c.
+// A workgroup can publish column c + 2 into a buffer only after every
+// workgroup published column c + 1, which each did after reading column c,
+// so the alternating buffers are never overwritten while read.
+template <int RPT>
+__global__ __launch_bounds__(256)
+void k_qr_panel(int m, double *__restrict__ A, int64_t lda, int k0, int nb,
+                double *__restrict__ rdiag, uint64_t *__restrict__ slots,
+                int32_t *__restrict__ status, uint64_t spin_ticks)
+{
+    constexpr int R = 256 * RPT;
+    constexpr int GR = 32 * RPT;                // rows per group of 32 threads
+    constexpr int KMAX = QRP_MAXP / 8;          // workgroups read per thread
+    extern __shared__ __align__(16) double qx[];    // R x QRP_LS
+    __shared__ double red[8][QR_NB], red2[8][QR_NB];
+    __shared__ double tot[QR_NB], drow[QR_NB], fac[QR_NB];
+    __shared__ int s_abort;
+    const int t = threadIdx.x, w = blockIdx.x, P = gridDim.x;
+    const int r0 = k0 + w * R;
+    for (int i = t; i < R * QR_NB; i += 256) {
+        const int lr = i / QR_NB, j = i % QR_NB;
+        const double v = A[(int64_t)min(r0 + lr, m - 1) * lda + k0 + min(j, nb - 1)];
+        qx[lr * QRP_LS + j] = (r0 + lr < m && j < nb) ? v : 0.0;
+    }
+    if (t == 0) s_abort = 0;
+    __syncthreads();
+    const int g = t >> 5, jt = t & 31;
+    for (int c = 0; c < nb; ++c) {
+        const int gc = k0 + c;
+        const uint32_t tag = (uint32_t)gc;
+        const int buf = c & 1;
+        // 1. this workgroup's partial sums
+        {
+            double acc = 0.0;
+            for (int q = 0; q < GR; ++q) {
+                const int lr = g * GR + q;
+                const double x = r0 + lr >= gc ? qx[lr * QRP_LS + c] : 0.0;
+                acc = fma(x, qx[lr * QRP_LS + jt], acc);
+            }
+            red[g][jt] = acc;
+        }
+        __syncthreads();
+        // 2. publish
+        if (t < QR_NB && t >= c && t < nb) {
+            double sq = 0.0;
+            for (int q = 0; q < 8; ++q) sq += red[q][t];
+            uint64_t *slot = slots + ((size_t)buf * QRP_MAXP + w) * QRP_SW;
+            put_tagged(slot + 2 * t, tag, sq);
+            if (w == 0) put_tagged(slot + 2 * QR_NB + 2 * t, tag, qx[c * QRP_LS + t]);
+        }
+        // 3. gather: loads clamped to words that will carry this tag, so every
+        // wait ends; out-of-range ones are not summed
+        {
+            const int jc = min(max(jt, c), nb - 1);
+            const bool jon = jt >= c && jt < nb;
+            uint64_t v[2 * KMAX];
+#pragma unroll
+            for (int k = 0; k < KMAX; ++k) {
+                const int ww = min(g + 8 * k, P - 1);
+                const uint64_t *src = slots + ((size_t)buf * QRP_MAXP + ww) * QRP_SW + 2 * jc;
+                v[2 * k] = ld_dev(src);
+                v[2 * k + 1] = ld_dev(src + 1);
+            }
+            uint64_t dv[2] = {0, 0};
+            const uint64_t *dsrc = slots + (size_t)buf * QRP_MAXP * QRP_SW + 2 * QR_NB + 2 * jc;
+            if (g == 0) { dv[0] = ld_dev(dsrc); dv[1] = ld_dev(dsrc + 1); }
+            bool ok = true;
+            double sum = 0.0;
+#pragma unroll
+            for (int k = 0; k < KMAX; ++k) {
+                const int ww = min(g + 8 * k, P - 1);
+                const uint64_t *src = slots + ((size_t)buf * QRP_MAXP + ww) * QRP_SW + 2 * jc;
+                ok = ok && wait_tagged(src, tag, v[2 * k], status, spin_ticks) &&
+                     wait_tagged(src + 1, tag, v[2 * k + 1], status, spin_ticks);
+                const double d = __longlong_as_double(
+                    (long long)((v[2 * k] & 0xffffffffull) | (v[2 * k + 1] << 32)));
+                if (g + 8 * k < P) sum += d;
+            }
+            if (g == 0) {
+                ok = ok && wait_tagged(dsrc, tag, dv[0], status, spin_ticks) &&
+                     wait_tagged(dsrc + 1, tag, dv[1], status, spin_ticks);
+                drow[jt] = jon ? __longlong_as_double((long long)((dv[0] & 0xffffffffull) |
+                                                                  (dv[1] << 32)))
+                               : 0.0;
+            }
+            red2[g][jt] = jon ? sum : 0.0;
+            if (!ok) s_abort = 1;
+        }
+        __syncthreads();
+        if (s_abort) break;
+        if (t < QR_NB) {
+            double s2 = 0.0;
+            for (int q = 0; q < 8; ++q) s2 += red2[q][t];
+            tot[t] = s2;
+        }
+        __syncthreads();
+        // 4. the reflection
+        const double xc = drow[c];
+        const double av = xc > 0.0 ? -sqrt(tot[c]) : sqrt(tot[c]);
+        const double uc = xc - av;
+        if (t < QR_NB)
+            fac[t] = (av != 0.0 && t > c && t < nb) ? (tot[t] - av * drow[t]) / (av * uc) : 0.0;
+        __syncthreads();
+        if (av != 0.0) {
+            for (int q = 0; q < RPT; ++q) {
+                const int lr = t + 256 * q, gr = r0 + lr;
+                if (gr < gc || gr >= m) continue;
+                double *row = qx + lr * QRP_LS;
+                const double u = gr == gc ? uc : row[c];
+                for (int j = c + 1; j < nb; ++j) row[j] = fma(fac[j], u, row[j]);
+                if (gr == gc) row[c] = uc;
+            }
+        }
+        if (w == 0 && t == 0) rdiag[gc] = av;
+        __syncthreads();
+    }
+    for (int i = t; i < R * QR_NB; i += 256) {
+        const int lr = i / QR_NB, j = i % QR_NB;
+        if (r0 + lr < m && j < nb) A[(int64_t)(r0 + lr) * lda + k0 + j] = qx[lr * QRP_LS + j];
+    }
+}
 
 // Row r of the panel's current column is handled by thread (r - r0) of the
 // grid (rows beyond the grid's size wrap by the grid size).
@@ -318,6 +488,7 @@ namespace {
 // Workspace of one QR call.
 struct QrWs {
     double *V, *G, *T, *Tt, *W, *W2, *npart, *dpart, *scal, *P;
+    uint64_t *slots;
 };
 
 int qr_ws(la_ctx *ctx, int64_t m, int64_t n, int64_t wcols, QrWs &w)
@@ -329,8 +500,9 @@ int qr_ws(la_ctx *ctx, int64_t m, int64_t n, int64_t wcols, QrWs &w)
                  bp = al((size_t)QR_PARTS * 8), bd = al((size_t)QR_PARTS * QR_NB * 8),
                  bs = al((size_t)2 * std::max<int64_t>(dc, 1) * 8),
                  bpk = al((size_t)VT_MAXCHUNK * QR_NB * std::max<int64_t>(std::max<int64_t>(wcols, 1), QR_NB) * 8);
+    const size_t bsl = al((size_t)2 * QRP_MAXP * QRP_SW * 8);
     void *base = nullptr;
-    int st = ws_get(ctx, WS_QR, bv + 3 * bg + 2 * bw + bp + bd + bs + bpk, &base);
+    int st = ws_get(ctx, WS_QR, bv + 3 * bg + 2 * bw + bp + bd + bs + bpk + bsl, &base);
     if (st) return st;
     char *p = (char *)base;
     w.V = (double *)p; p += bv;
@@ -342,7 +514,8 @@ int qr_ws(la_ctx *ctx, int64_t m, int64_t n, int64_t wcols, QrWs &w)
     w.npart = (double *)p; p += bp;
     w.dpart = (double *)p; p += bd;
     w.scal = (double *)p; p += bs;
-    w.P = (double *)p;
+    w.P = (double *)p; p += bpk;
+    w.slots = (uint64_t *)p;
     return LA_OK;
 }
 
@@ -404,9 +577,34 @@ int geqrf_device(la_ctx *ctx, int64_t m, int64_t n, double *A, int64_t lda, doub
     QrWs w;
     int st = qr_ws(ctx, m, n, n, w);
     if (st) return st;
+    // cooperative panels (LAHIP_QR_COLUMN_KERNELS=1 selects the per-column
+    // launches everywhere, for testing); the exchange slots start with a tag
+    // no column uses
+    const char *ck = getenv("LAHIP_QR_COLUMN_KERNELS");
+    const bool coop = !(ck && atoi(ck) != 0);
+    const uint64_t ticks = spin_limit_ticks(ctx);
+    if (coop && (st = fill_u64(ctx, w.slots, (int64_t)2 * QRP_MAXP * QRP_SW,
+                               0xffffffff00000000ull)))
+        return st;
     for (int64_t k0 = 0; k0 < dc; k0 += QR_NB) {
         const int nb = (int)std::min<int64_t>(QR_NB, dc - k0);
         const int64_t cend = k0 + nb;
+        const int64_t rows = m - k0;
+        const int rpt = rows <= (int64_t)256 * QRP_MAXP ? 1 : rows <= (int64_t)512 * QRP_MAXP ? 2 : 0;
+        if (coop && rpt > 0) {
+            const int Pq = (int)ceil_div(rows, (int64_t)256 * rpt);
+            const void *fn = rpt == 1 ? (const void *)k_qr_panel<1> : (const void *)k_qr_panel<2>;
+            const size_t shmem = (size_t)256 * rpt * QRP_LS * sizeof(double);
+            static size_t lds_max[3] = {0, 0, 0};
+            if (!lds_max[rpt]) lds_max[rpt] = allow_dynamic_lds(fn);
+            if (shmem > lds_max[rpt]) return set_error(ctx, LA_ERR_ARG, "qr panel: LDS");
+            int mi = (int)m, k0i = (int)k0;
+            int32_t *stp = ctx->d_scratch + SCR_ERR;
+            uint64_t tk = ticks;
+            void *args[] = {&mi, &A, &lda, &k0i, (void *)&nb, &rdiag, &w.slots, &stp, &tk};
+            LA_HIP_CHECK(ctx, hipLaunchCooperativeKernel(fn, dim3(Pq), dim3(256), args,
+                                                         (unsigned)shmem, ctx->stream));
+        } else {
         // column by column inside the panel; the first column's norm first
         int P = grid_rows(m - k0);
         hipLaunchKernelGGL(k_qr_norm_partial, dim3(P), dim3(256), 0, ctx->stream, (int)m,
@@ -424,6 +622,7 @@ int geqrf_device(la_ctx *ctx, int64_t m, int64_t n, double *A, int64_t lda, doub
                                (const double *)w.scal, rdiag, w.npart);
             LA_HIP_CHECK(ctx, hipGetLastError());
             P = Pc;
+        }
         }
         // the block's reflections on the trailing columns
         if (cend < n) {

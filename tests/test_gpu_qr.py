@@ -155,3 +155,28 @@ def test_qr_device_entry_points(tctx, torch_cuda):
     Xw = torch.empty((5, 1), dtype=torch.float64, device="cuda")
     assert tctx.qrs_dev(W, RW, torch.ones((3, 1), dtype=torch.float64, device="cuda"),
                         Xw) == la_hip.LA_ERR_SHAPE
+
+
+@pytest.mark.parametrize("shape", [(300, 300), (513, 257), (2048, 384)])
+def test_qr_column_kernel_path(ctx, shape, monkeypatch):
+    """The per-column launches, which factor panels of more than 65536 rows
+    (LAHIP_QR_COLUMN_KERNELS=1 selects them for every panel), against the
+    oracle to the same bar as the cooperative panel."""
+    monkeypatch.setenv("LAHIP_QR_COLUMN_KERNELS", "1")
+    mm, nn = shape
+    a = oracle.normal(4243, mm * 7 + nn, shape)
+    qr, rd = ctx.qr_new(a)
+    qr_o, rd_o = oracle.qr_new(a)
+    scale = np.linalg.norm(a)
+    assert np.array_equal(np.sign(rd), np.sign(rd_o))
+    assert _nrel(rd, rd_o, scale) <= 1e-12
+    assert _nrel(qr, qr_o, scale) <= 1e-12
+
+
+def test_qr_panel_deterministic(ctx):
+    """Every sum of the cooperative panel is taken in a fixed order: two runs
+    give the same bits."""
+    a = oracle.normal(4244, 1, (3000, 700))
+    qr1, rd1 = ctx.qr_new(a)
+    qr2, rd2 = ctx.qr_new(a)
+    assert np.array_equal(qr1, qr2) and np.array_equal(rd1, rd2)
