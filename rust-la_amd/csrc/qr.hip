@@ -80,7 +80,7 @@ __device__ inline bool wait_tagged(const uint64_t *p, uint32_t tag, uint64_t &v,
 // A workgroup can publish column c + 2 into a buffer only after every
 // workgroup published column c + 1, which each did after reading column c,
 // so the alternating buffers are never overwritten while read.
-template <int RPT>
+template <int RPT, int KMAX>
 __global__ __launch_bounds__(256)
 void k_qr_panel(int m, double *__restrict__ A, int64_t lda, int k0, int nb,
                 double *__restrict__ rdiag, uint64_t *__restrict__ slots,
@@ -88,7 +88,7 @@ void k_qr_panel(int m, double *__restrict__ A, int64_t lda, int k0, int nb,
 {
     constexpr int R = 256 * RPT;
     constexpr int GR = 32 * RPT;                // rows per group of 32 threads
-    constexpr int KMAX = QRP_MAXP / 8;          // workgroups read per thread
+    // KMAX >= ceil(P / 8): workgroups read per thread
     extern __shared__ __align__(16) double qx[];    // R x QRP_LS
     __shared__ double red[8][QR_NB], red2[8][QR_NB];
     __shared__ double tot[QR_NB], drow[QR_NB], fac[QR_NB];
@@ -372,10 +372,18 @@ void k_qr_vtc_partial(int rows, int nb, int cols, int rch, const double *__restr
         }
         __syncthreads();
         const double *cp = C + (int64_t)rs * ldc + (okc ? col : 0);
-        for (int rr = 0; rr < rn; ++rr) {
-            const double c = okc ? cp[(int64_t)rr * ldc] : 0.0;
+        // eight rows of C in flight per step (clamped loads, uniform guard)
+        for (int rr0 = 0; rr0 < rn; rr0 += 8) {
+            double cv[8];
 #pragma unroll
-            for (int j = 0; j < 16; ++j) acc[j] = fma(vs[rr][jh + j], c, acc[j]);
+            for (int u = 0; u < 8; ++u) cv[u] = cp[(int64_t)min(rr0 + u, rn - 1) * ldc];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                if (rr0 + u < rn) {
+#pragma unroll
+                    for (int j = 0; j < 16; ++j) acc[j] = fma(vs[rr0 + u][jh + j], cv[u], acc[j]);
+                }
+            }
         }
         __syncthreads();
     }
@@ -421,25 +429,46 @@ void k_qr_larft(int nb, const double *__restrict__ G, const double *__restrict__
                 const double *__restrict__ diagu, int64_t ldu, double *__restrict__ T,
                 double *__restrict__ Tt)
 {
-    __shared__ double sT[QR_NB][QR_NB];
-    __shared__ double w[QR_NB];
+    __shared__ double sT[QR_NB][QR_NB + 1], sG[QR_NB][QR_NB + 1];
+    __shared__ double stau[QR_NB];
     const int t = threadIdx.x;
-    for (int e = t; e < QR_NB * QR_NB; e += 64) sT[e / QR_NB][e % QR_NB] = 0.0;
-    __syncthreads();
-    for (int j = 0; j < nb; ++j) {
-        const double a = rd[j], u = diagu[(int64_t)j * ldu + j];
-        const double tau = (a != 0.0) ? -1.0 / (a * u) : 0.0;
-        // w = T[0:j, 0:j] G[0:j, j]
-        if (t < j) {
-            double s = 0.0;
-            for (int q = t; q < j; ++q) s += sT[t][q] * G[q * nb + j];
-            w[t] = s;
-        }
-        __syncthreads();
-        if (t < j) sT[t][j] = -tau * w[t];
-        if (t == 0) sT[j][j] = tau;
-        __syncthreads();
+    // G, rdiag and the diagonal of u staged in LDS first: the recurrence
+    // below then issues no global load
+    for (int e = t; e < QR_NB * QR_NB; e += 64) {
+        const int i = e / QR_NB, j = e % QR_NB;
+        sT[i][j] = 0.0;
+        const double g = G[min(i, nb - 1) * nb + min(j, nb - 1)];
+        sG[i][j] = (i < nb && j < nb) ? g : 0.0;
     }
+    if (t < QR_NB) {
+        const int jc = min(t, nb - 1);
+        const double a = rd[jc], u = diagu[(int64_t)jc * ldu + jc];
+        stau[t] = (t < nb && a != 0.0) ? -1.0 / (a * u) : 0.0;
+    }
+    __syncthreads();
+    // thread t < 32 keeps row t of T in registers; per column j it forms
+    // w_t = sum_{q = t}^{j-1} T[t][q] G[q][j] (q ascending, as the textbook
+    // recurrence) from LDS broadcasts of G's column, with no barrier
+    double tr[QR_NB];
+#pragma unroll
+    for (int q = 0; q < QR_NB; ++q) tr[q] = 0.0;
+#pragma unroll
+    for (int j = 0; j < QR_NB; ++j) {
+        if (j < nb) {
+            double s = 0.0;
+#pragma unroll
+            for (int q = 0; q < j; ++q) {
+                const double g = sG[q][j];
+                if (q >= t) s += tr[q] * g;
+            }
+            if (t < j) tr[j] = -stau[j] * s;
+            else if (t == j) tr[j] = stau[j];
+        }
+    }
+    if (t < QR_NB)
+#pragma unroll
+        for (int q = 0; q < QR_NB; ++q) sT[t][q] = tr[q];
+    __syncthreads();
     for (int e = t; e < nb * nb; e += 64) {
         const int i = e / nb, j = e % nb;
         T[e] = sT[i][j];
@@ -593,11 +622,21 @@ int geqrf_device(la_ctx *ctx, int64_t m, int64_t n, double *A, int64_t lda, doub
         const int rpt = rows <= (int64_t)256 * QRP_MAXP ? 1 : rows <= (int64_t)512 * QRP_MAXP ? 2 : 0;
         if (coop && rpt > 0) {
             const int Pq = (int)ceil_div(rows, (int64_t)256 * rpt);
-            const void *fn = rpt == 1 ? (const void *)k_qr_panel<1> : (const void *)k_qr_panel<2>;
+            const int kneed = (Pq + 7) / 8;
+            const int ki = kneed <= 1 ? 0 : kneed <= 2 ? 1 : kneed <= 4 ? 2 : kneed <= 8 ? 3 : 4;
+            static const void *const fns[2][5] = {
+                {(const void *)k_qr_panel<1, 1>, (const void *)k_qr_panel<1, 2>,
+                 (const void *)k_qr_panel<1, 4>, (const void *)k_qr_panel<1, 8>,
+                 (const void *)k_qr_panel<1, 16>},
+                {(const void *)k_qr_panel<2, 1>, (const void *)k_qr_panel<2, 2>,
+                 (const void *)k_qr_panel<2, 4>, (const void *)k_qr_panel<2, 8>,
+                 (const void *)k_qr_panel<2, 16>}};
+            const void *fn = fns[rpt - 1][ki];
             const size_t shmem = (size_t)256 * rpt * QRP_LS * sizeof(double);
-            static size_t lds_max[3] = {0, 0, 0};
-            if (!lds_max[rpt]) lds_max[rpt] = allow_dynamic_lds(fn);
-            if (shmem > lds_max[rpt]) return set_error(ctx, LA_ERR_ARG, "qr panel: LDS");
+            static size_t lds_max[2][5] = {};
+            size_t &lm = lds_max[rpt - 1][ki];
+            if (!lm) lm = allow_dynamic_lds(fn);
+            if (shmem > lm) return set_error(ctx, LA_ERR_ARG, "qr panel: LDS");
             int mi = (int)m, k0i = (int)k0;
             int32_t *stp = ctx->d_scratch + SCR_ERR;
             uint64_t tk = ticks;
