@@ -851,7 +851,7 @@ void k_trsm_unit_lower(const T *Linv, int nb, T *B, int64_t ldb, int ncols)
 // kernel's inverse, unit diagonal implied; same operation order as
 // k_trsm_unit_lower), then X_r -= L_rb X_b for every panel r below it, with
 // L_rb read from the factored block (Lblk = A + K0 * lda + K0).  Replaces
-// per block four TRSM and three GEMM launches.
+// per block four TRSM and three GEMM launches.  CW = 8 (FP32) or 16 (FP64).
 template <typename T, int CW>
 __global__ __launch_bounds__(256)
 void k_u12_block(const T *__restrict__ Lblk, int64_t lda, const T *__restrict__ linv, int nbo,
@@ -1113,15 +1113,13 @@ static int launch_trsm(la_ctx *ctx, const T *Linv, int nb, T *B, int64_t ldb, in
     return LA_OK;
 }
 
-template <typename T>
-static int launch_u12_block(la_ctx *ctx, const T *Lblk, int64_t lda, const T *linv, int nbo,
-                            T *B, int64_t ldb, int64_t ncols)
+template <typename T, int CW>
+static int launch_u12_cw(la_ctx *ctx, const T *Lblk, int64_t lda, const T *linv, int nbo,
+                         T *B, int64_t ldb, int64_t ncols)
 {
-    constexpr int CW = sizeof(T) == 8 ? 16 : 32;
     const void *fn = (const void *)k_u12_block<T, CW>;
     const size_t shmem = ((size_t)NB_OUT * (CW + 1) + 2 * 64 * 65) * sizeof(T);
-    static size_t lds_max[2] = {0, 0};
-    size_t &lm = lds_max[sizeof(T) == 8 ? 0 : 1];
+    static size_t lm = 0;
     if (!lm) lm = allow_dynamic_lds(fn);
     if (shmem > lm) return set_error(ctx, LA_ERR_ARG, "getrf: U12 block exceeds LDS");
     ProfScope ps(ctx, LA_K_TRSM, (double)nbo * nbo * ncols,
@@ -1130,6 +1128,20 @@ static int launch_u12_block(la_ctx *ctx, const T *Lblk, int64_t lda, const T *li
                        dim3(256), shmem, ctx->stream, Lblk, lda, linv, nbo, B, ldb, (int)ncols);
     LA_HIP_CHECK(ctx, hipGetLastError());
     return LA_OK;
+}
+
+template <typename T>
+static int launch_u12_block(la_ctx *ctx, const T *Lblk, int64_t lda, const T *linv, int nbo,
+                            T *B, int64_t ldb, int64_t ncols)
+{
+    // columns per workgroup, measured at n = 16384 with look-ahead (FP32-IR
+    // 92.0 / 92.4 / 93.2 ms, FP64 LU 119.0 / 114.9 / 116.8 ms for 8 / 16 / 32):
+    // narrow workgroups shorten the U12 of the next block's 256 columns,
+    // which the next panel waits for
+    if constexpr (sizeof(T) == 4)
+        return launch_u12_cw<T, 8>(ctx, Lblk, lda, linv, nbo, B, ldb, ncols);
+    else
+        return launch_u12_cw<T, 16>(ctx, Lblk, lda, linv, nbo, B, ldb, ncols);
 }
 
 // Trailing updates: native MFMA for T, or FP32 data with BF16/FP16-rounded
@@ -1261,7 +1273,11 @@ int getrf_core(la_ctx *ctx, int64_t m, int64_t n, T *A, int64_t lda, int64_t *d_
     auto al = [](size_t x) { return ws_align(x); };
     void *base = ext_ws;
     int st;
-    const size_t ring_bytes = K0la < kmax ? RING * al(map_bytes) : 0;
+    // look-ahead: swap maps and L11^-1 blocks of two outer blocks, since the
+    // update stream still applies block j (row moves, U12) while the panel
+    // stream factors block j+1
+    const size_t linv_bytes = al((size_t)(NB_OUT / NB_IN) * NB_IN * NB_IN * sizeof(T));
+    const size_t ring_bytes = K0la < kmax ? RING * al(map_bytes) + linv_bytes : 0;
     if (!base && (st = ws_get(ctx, WS_PANEL, panel_ws_bytes(sizeof(T)) + ring_bytes, &base)))
         return st;
     PanelWS pw;
@@ -1271,9 +1287,10 @@ int getrf_core(la_ctx *ctx, int64_t m, int64_t n, T *A, int64_t lda, int64_t *d_
     pw.diagbuf = b; b += al(diag_bytes);
     pw.swapmap = (int64_t *)b; b += al(map_bytes);
     T *linv_ring = (T *)b;
-    b += al((size_t)(NB_OUT / NB_IN) * NB_IN * NB_IN * sizeof(T));
+    b += linv_bytes;
     int64_t *swap_ring = (int64_t *)b;
     const size_t ring_stride = al(map_bytes) / sizeof(int64_t);
+    T *linv_ring2 = (T *)(b + RING * al(map_bytes));      // look-ahead only
 
     // step tag 0xffffffff never occurs: no stale word can validate
     const int64_t nslot_words = 2 * PANEL_MAXP * CAND_WORDS;
@@ -1288,11 +1305,11 @@ int getrf_core(la_ctx *ctx, int64_t m, int64_t n, T *A, int64_t lda, int64_t *d_
     // block's other columns, in-block updates).  outside_moves: also apply
     // each panel's moves to the columns outside the block and to perm.
     auto factor_block = [&](int64_t K0, int64_t nbo, int64_t *maps, size_t map_stride,
-                            int mask_cus, bool outside_moves) -> int {
+                            T *linvs, int mask_cus, bool outside_moves) -> int {
         for (int64_t k = K0; k < K0 + nbo; k += NB_IN) {
             const int nbi = (int)std::min<int64_t>(NB_IN, K0 + nbo - k);
             T *Ap = A + k * lda + k;
-            T *linv = linv_ring + (size_t)((k - K0) / NB_IN) * NB_IN * NB_IN;
+            T *linv = linvs + (size_t)((k - K0) / NB_IN) * NB_IN * NB_IN;
             int64_t *smap = maps + (size_t)((k - K0) / NB_IN) * map_stride;
             int s;
             if ((s = launch_panel<T>(ctx, Ap, lda, (int)(m - k), nbi, (int)k, d_ipiv, pw, smap,
@@ -1319,14 +1336,15 @@ int getrf_core(la_ctx *ctx, int64_t m, int64_t n, T *A, int64_t lda, int64_t *d_
         return LA_OK;
     };
     // U12 = L11^-1 A12 for block K0 on columns [c0, c0 + ncol) (the block's rows).
-    auto block_u12 = [&](int64_t K0, int64_t nbo, int64_t c0, int64_t ncol) -> int {
+    auto block_u12 = [&](int64_t K0, int64_t nbo, int64_t c0, int64_t ncol,
+                         const T *linvs) -> int {
         if (ncol <= 0) return LA_OK;
         if (upd_prec != LA_PREC_BF16 && upd_prec != LA_PREC_FP16)
-            return launch_u12_block<T>(ctx, A + K0 * lda + K0, lda, linv_ring, (int)nbo,
+            return launch_u12_block<T>(ctx, A + K0 * lda + K0, lda, linvs, (int)nbo,
                                        A + K0 * lda + c0, lda, ncol);
         for (int64_t kb = K0; kb < K0 + nbo; kb += NB_IN) {
             const int nbi = (int)std::min<int64_t>(NB_IN, K0 + nbo - kb);
-            const T *linv_kb = linv_ring + (size_t)((kb - K0) / NB_IN) * NB_IN * NB_IN;
+            const T *linv_kb = linvs + (size_t)((kb - K0) / NB_IN) * NB_IN * NB_IN;
             int s;
             if ((s = launch_trsm<T>(ctx, linv_kb, nbi, A + kb * lda + c0, lda, ncol))) return s;
             const int64_t below = K0 + nbo - (kb + nbi);
@@ -1348,9 +1366,9 @@ int getrf_core(la_ctx *ctx, int64_t m, int64_t n, T *A, int64_t lda, int64_t *d_
     // plain schedule
     for (int64_t K0 = 0; K0 < K0la; K0 += NB_OUT) {
         const int64_t nbo = std::min<int64_t>(NB_OUT, kmax - K0);
-        if ((st = factor_block(K0, nbo, pw.swapmap, 0, 0, true))) return st;
+        if ((st = factor_block(K0, nbo, pw.swapmap, 0, linv_ring, 0, true))) return st;
         if (n - (K0 + nbo) > 0) {
-            if ((st = block_u12(K0, nbo, K0 + nbo, n - (K0 + nbo)))) return st;
+            if ((st = block_u12(K0, nbo, K0 + nbo, n - (K0 + nbo), linv_ring))) return st;
             if ((st = block_gemm(K0, nbo, K0 + nbo, n - (K0 + nbo)))) return st;
         }
     }
@@ -1388,10 +1406,12 @@ int getrf_core(la_ctx *ctx, int64_t m, int64_t n, T *A, int64_t lda, int64_t *d_
     for (int64_t K0 = K0la; K0 < kmax; K0 += NB_OUT, parity ^= 1) {
         const int64_t nbo = std::min<int64_t>(NB_OUT, kmax - K0);
         int64_t *maps = swap_ring + (size_t)parity * (NB_OUT / NB_IN) * ring_stride;
+        T *linvs = parity ? linv_ring2 : linv_ring;
         const int npan = (int)ceil_div(nbo, (int64_t)NB_IN);
         sw.use(la->panel);
         if (K0 > K0la) LA_HIP_CHECK(ctx, hipStreamWaitEvent(la->panel, ev_la, 0));
-        if ((st = factor_block(K0, nbo, maps, ring_stride, la->panel_cus, false))) return st;
+        if ((st = factor_block(K0, nbo, maps, ring_stride, linvs, la->panel_cus, false)))
+            return st;
         LA_HIP_CHECK(ctx, hipEventRecord(ev_pan, la->panel));
         const bool shared_prev = share_pending;
         if (share_pending) {
@@ -1413,7 +1433,7 @@ int getrf_core(la_ctx *ctx, int64_t m, int64_t n, T *A, int64_t lda, int64_t *d_
             if ((st = launch_rowmoves<T>(ctx, A + c1, lda, nla, 0, 0, maps, nullptr, npan,
                                          (int64_t)ring_stride)))
                 return st;
-            if ((st = block_u12(K0, nbo, c1, nla))) return st;
+            if ((st = block_u12(K0, nbo, c1, nla, linvs))) return st;
             if ((st = block_gemm(K0, nbo, c1, nla))) return st;
         }
         LA_HIP_CHECK(ctx, hipEventRecord(ev_la, la->update));
@@ -1421,7 +1441,7 @@ int getrf_core(la_ctx *ctx, int64_t m, int64_t n, T *A, int64_t lda, int64_t *d_
         if ((st = launch_rowmoves<T>(ctx, A, lda, n, K0, c1 + nla, maps, d_perm, npan,
                                      (int64_t)ring_stride)))
             return st;
-        if ((st = block_u12(K0, nbo, c1 + nla, ncol - nla))) return st;
+        if ((st = block_u12(K0, nbo, c1 + nla, ncol - nla, linvs))) return st;
         const int64_t nbulk = ncol - nla, rows = m - c1;
         int64_t nshare = 0;
         if (nbulk > 2 * NB_OUT && rows > 0 && K0 + 2 * NB_OUT < kmax) {
