@@ -81,3 +81,22 @@ def test_lookahead_refinement_bit_identical(tctx, torch_cuda, prec, n, kappa):
     assert st1 == st2 == la_hip.LA_OK
     assert it1 == it2 and e1 == e2
     assert torch.equal(X1, X2)
+
+
+@pytest.mark.parametrize("shape", [(3000, 3000), (8448, 8448)])
+def test_lookahead_repeatable(tctx, torch_cuda, shape):
+    """Four look-ahead factorisations of one matrix all equal the
+    single-stream factor bit for bit.  The two streams share buffers only
+    through events (swap maps and L11^-1 blocks alternate between outer
+    blocks), so the result may not depend on how the streams' kernels
+    happen to interleave from run to run."""
+    torch = torch_cuda
+    print(f"look-ahead repeat {shape}", flush=True)
+    A0 = torch.empty(shape, dtype=torch.float64, device="cuda")
+    tctx.fill_normal_dev(777, shape[0], A0)
+    torch.cuda.synchronize()
+    A1, p1, _ = _getrf(tctx, torch, A0, False)
+    for _ in range(4):
+        A2, p2, i2 = _getrf(tctx, torch, A0, True)
+        assert i2[1].item() == 0
+        assert torch.equal(p1, p2) and torch.equal(A1, A2)
