@@ -77,6 +77,10 @@ __device__ inline bool wait_tagged(const uint64_t *p, uint32_t tag, uint64_t &v,
 //      g, g + 8, ...) and sums them in one fixed order;
 //   4. a = -sign(x_c)|x|, u_c = x_c - a, f_j = (q_j - a A[gc][j]) / (a u_c),
 //      and its rows r >= gc get A[r][j] += f_j u_r (u_gc = u_c), A[gc][c] = u_c.
+// With want_vg the kernel also writes its rows of V (the Householder vectors,
+// as k_qr_make_v builds them) and its partial Gram matrix V_w^T V_w (on MFMA,
+// a fixed order) as chunk w of the partial products that k_qr_vtc_reduce
+// adds up, so the block's V^T V needs no pass over V.
 // A workgroup can publish column c + 2 into a buffer only after every
 // workgroup published column c + 1, which each did after reading column c,
 // so the alternating buffers are never overwritten while read.
@@ -84,14 +88,15 @@ template <int RPT, int KMAX>
 __global__ __launch_bounds__(256)
 void k_qr_panel(int m, double *__restrict__ A, int64_t lda, int k0, int nb,
                 double *__restrict__ rdiag, uint64_t *__restrict__ slots,
-                int32_t *__restrict__ status, uint64_t spin_ticks)
+                int32_t *__restrict__ status, uint64_t spin_ticks, double *__restrict__ V,
+                double *__restrict__ gpart, int want_vg)
 {
     constexpr int R = 256 * RPT;
     constexpr int GR = 32 * RPT;                // rows per group of 32 threads
     // KMAX >= ceil(P / 8): workgroups read per thread
     extern __shared__ __align__(16) double qx[];    // R x QRP_LS
     __shared__ double red[8][QR_NB], red2[8][QR_NB];
-    __shared__ double tot[QR_NB], drow[QR_NB], fac[QR_NB];
+    __shared__ double tot[QR_NB], drow[QR_NB], fac[QR_NB], s_av[QR_NB];
     __shared__ int s_abort;
     const int t = threadIdx.x, w = blockIdx.x, P = gridDim.x;
     const int r0 = k0 + w * R;
@@ -101,6 +106,7 @@ void k_qr_panel(int m, double *__restrict__ A, int64_t lda, int k0, int nb,
         qx[lr * QRP_LS + j] = (r0 + lr < m && j < nb) ? v : 0.0;
     }
     if (t == 0) s_abort = 0;
+    if (t < QR_NB) s_av[t] = 0.0;
     __syncthreads();
     const int g = t >> 5, jt = t & 31;
     for (int c = 0; c < nb; ++c) {
@@ -190,11 +196,44 @@ void k_qr_panel(int m, double *__restrict__ A, int64_t lda, int k0, int nb,
             }
         }
         if (w == 0 && t == 0) rdiag[gc] = av;
+        if (t == 0) s_av[c] = av;
         __syncthreads();
     }
     for (int i = t; i < R * QR_NB; i += 256) {
         const int lr = i / QR_NB, j = i % QR_NB;
         if (r0 + lr < m && j < nb) A[(int64_t)(r0 + lr) * lda + k0 + j] = qx[lr * QRP_LS + j];
+    }
+    if (!want_vg || s_abort) return;
+    // V in place: column j zero above row k0 + j and where a_j = 0 (rows past
+    // m and columns past nb are zero already)
+    __syncthreads();
+    for (int i = t; i < R * QR_NB; i += 256) {
+        const int lr = i / QR_NB, j = i % QR_NB;
+        if (!(r0 + lr >= k0 + j && s_av[j] != 0.0)) qx[lr * QRP_LS + j] = 0.0;
+    }
+    __syncthreads();
+    for (int i = t; i < R * QR_NB; i += 256) {
+        const int lr = i / QR_NB, j = i % QR_NB;
+        if (r0 + lr < m && j < nb) V[(int64_t)(r0 + lr - k0) * nb + j] = qx[lr * QRP_LS + j];
+    }
+    // V_w^T V_w on v_mfma_f64_16x16x4: wave wv owns the 16 x 16 tile
+    // (16 ti, 16 tj) and runs K over the workgroup's rows four at a time;
+    // lane l supplies A[l%16][l/16] = V[k][16 ti + l%16] and
+    // B[l/16][l%16] = V[k][16 tj + l%16], k = row l/16 of the step, and
+    // receives D[l/16 + 4 r][l%16]
+    {
+        const int lane = t & 63, wv = t >> 6, ti = wv >> 1, tj = wv & 1;
+        d4_t acc = {0.0, 0.0, 0.0, 0.0};
+        for (int k4 = 0; k4 < R; k4 += 4) {
+            const double* rowp = qx + (k4 + (lane >> 4)) * QRP_LS;
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(rowp[16 * ti + (lane & 15)],
+                                                       rowp[16 * tj + (lane & 15)], acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int gi = 16 * ti + (lane >> 4) + 4 * r, gj = 16 * tj + (lane & 15);
+            if (gi < nb && gj < nb) gpart[((int64_t)w * QR_NB + gi) * nb + gj] = acc[r];
+        }
     }
 }
 
@@ -528,7 +567,8 @@ int qr_ws(la_ctx *ctx, int64_t m, int64_t n, int64_t wcols, QrWs &w)
                  bw = al((size_t)QR_NB * std::max<int64_t>(wcols, 1) * 8),
                  bp = al((size_t)QR_PARTS * 8), bd = al((size_t)QR_PARTS * QR_NB * 8),
                  bs = al((size_t)2 * std::max<int64_t>(dc, 1) * 8),
-                 bpk = al((size_t)VT_MAXCHUNK * QR_NB * std::max<int64_t>(std::max<int64_t>(wcols, 1), QR_NB) * 8);
+                 bpk = al(std::max((size_t)VT_MAXCHUNK * QR_NB * std::max<int64_t>(std::max<int64_t>(wcols, 1), QR_NB) * 8,
+                                   (size_t)QRP_MAXP * QR_NB * QR_NB * 8));
     const size_t bsl = al((size_t)2 * QRP_MAXP * QRP_SW * 8);
     void *base = nullptr;
     int st = ws_get(ctx, WS_QR, bv + 3 * bg + 2 * bw + bp + bd + bs + bpk + bsl, &base);
@@ -565,6 +605,15 @@ int qr_vtc(la_ctx *ctx, int64_t rows, int nb, int64_t cols, const double *C, int
     return LA_OK;
 }
 
+// T of a panel from its G = V^T V (in w.G); diagu: the panel's diagonal block.
+int qr_larft(la_ctx *ctx, int nb, const double *rd, const double *diagu, int64_t ldu, QrWs &w)
+{
+    hipLaunchKernelGGL(k_qr_larft, dim3(1), dim3(64), 0, ctx->stream, nb, (const double *)w.G,
+                       rd, diagu, ldu, w.T, w.Tt);
+    LA_HIP_CHECK(ctx, hipGetLastError());
+    return LA_OK;
+}
+
 // V and T of the panel at column k0 (width nb) of the packed factor.
 int qr_block(la_ctx *ctx, int64_t m, const double *QR, int64_t ldq, const double *rdiag,
              int64_t k0, int nb, QrWs &w)
@@ -575,10 +624,7 @@ int qr_block(la_ctx *ctx, int64_t m, const double *QR, int64_t ldq, const double
     LA_HIP_CHECK(ctx, hipGetLastError());
     int st = qr_vtc(ctx, rows, nb, nb, w.V, nb, w.G, w);
     if (st) return st;
-    hipLaunchKernelGGL(k_qr_larft, dim3(1), dim3(64), 0, ctx->stream, nb, (const double *)w.G,
-                       rdiag + k0, QR + k0 * ldq + k0, ldq, w.T, w.Tt);
-    LA_HIP_CHECK(ctx, hipGetLastError());
-    return LA_OK;
+    return qr_larft(ctx, nb, rdiag + k0, QR + k0 * ldq + k0, ldq, w);
 }
 
 // C (rows x cols, ld ldc, the rows k0.. of the operand) <- (I - V op(T) V^T) C
@@ -640,9 +686,23 @@ int geqrf_device(la_ctx *ctx, int64_t m, int64_t n, double *A, int64_t lda, doub
             int mi = (int)m, k0i = (int)k0;
             int32_t *stp = ctx->d_scratch + SCR_ERR;
             uint64_t tk = ticks;
-            void *args[] = {&mi, &A, &lda, &k0i, (void *)&nb, &rdiag, &w.slots, &stp, &tk};
+            int want = cend < n ? 1 : 0;
+            void *args[] = {&mi, &A, &lda, &k0i, (void *)&nb, &rdiag, &w.slots, &stp, &tk,
+                            &w.V, &w.P, &want};
             LA_HIP_CHECK(ctx, hipLaunchCooperativeKernel(fn, dim3(Pq), dim3(256), args,
                                                          (unsigned)shmem, ctx->stream));
+            if (want) {
+                // the block's V^T V from the panel's per-workgroup partials, then T
+                hipLaunchKernelGGL(k_qr_vtc_reduce, dim3((unsigned)ceil_div(nb * nb, 256)),
+                                   dim3(256), 0, ctx->stream, Pq, nb, nb, (const double *)w.P,
+                                   w.G);
+                LA_HIP_CHECK(ctx, hipGetLastError());
+                if ((st = qr_larft(ctx, nb, rdiag + k0, A + k0 * lda + k0, lda, w))) return st;
+                if ((st = qr_apply_block(ctx, m - k0, nb, n - cend, A + k0 * lda + cend, lda,
+                                         true, w)))
+                    return st;
+            }
+            continue;
         } else {
         // column by column inside the panel; the first column's norm first
         int P = grid_rows(m - k0);
