@@ -1248,9 +1248,13 @@ struct StreamSwap {
 // block j's row moves, U12 = L11^-1 A12 and update on block j+1's columns
 // (after which block j+1 may start), then block j's row moves on the columns
 // left of it and right of block j+1 (and the permutation vector), U12 and
-// the update there.  Every location is written by one stream
-// between two event waits, and row moves on a column keep their block order,
-// so the factor is identical to the plain schedule's.
+// the update there; while the updates are large, the panel stream takes the
+// rightmost columns of that update once block j+1 is factored (the share).
+// Buffers that block j's update still reads while block j+1 is factored
+// (swap maps, L11^-1 blocks) alternate between two copies by block parity.
+// Every location is written by one stream between two event waits, and row
+// moves on a column keep their block order, so the factor is identical to
+// the plain schedule's.
 template <typename T>
 int getrf_core(la_ctx *ctx, int64_t m, int64_t n, T *A, int64_t lda, int64_t *d_ipiv,
                uint64_t *d_perm, int32_t *d_status, int upd_prec, void *ext_ws)
