@@ -277,6 +277,71 @@ def cfg2(ctx, torch, reps):
                            "frac_of_fp64_peak": tf_gemm / PEAKS["fp64"]}}
 
 
+def next_rows(ctx, torch, reps):
+    """The rows next to the path, timed on the device: Matrix::inverse at
+    n = 8192 (LU, then the inverse by two MFMA TRSMs; (2/3 + 2) n^3 flop),
+    Matrix<f32>::simd_mul's FP32 product at 8192^3, and dense Cholesky +
+    one-RHS solve at n = 16384 ((1/3) n^3 + 2 n^2 flop)."""
+    out = {}
+    n = 8192
+    A = torch.empty((n, n), dtype=torch.float64, device="cuda")
+    ctx.fill_normal_dev(2025, 0, A)
+    W, X = torch.empty_like(A), torch.empty_like(A)
+    piv = torch.empty(n, dtype=torch.int64, device="cuda")
+    info = torch.zeros(4, dtype=torch.int32, device="cuda")
+    W.copy_(A)
+    ctx.inverse_dev(W, piv, X, info)
+    t = []
+    for _ in range(reps):
+        W.copy_(A)
+        ms, _ = timed(torch, lambda: ctx.inverse_dev(W, piv, X, info))
+        t.append(ms)
+    ms = sum(t) / len(t)
+    flop = (2.0 / 3.0 + 2.0) * n ** 3
+    # residual of the inverse on a few columns: |A X[:, j] - e_j|
+    cols = torch.arange(0, n, n // 8, device="cuda")
+    R = A @ X[:, cols]
+    R[cols, torch.arange(len(cols), device="cuda")] -= 1.0
+    out["inverse_n8192"] = {"ms": ms, "tflops": flop / (ms * 1e-3) / 1e12,
+                            "frac_of_fp64_peak": flop / (ms * 1e-3) / 1e12 / PEAKS["fp64"],
+                            "max_abs_residual_sampled": float(R.abs().max().item()),
+                            "status": int(info[1].item())}
+    del W, X, A, R
+    As = torch.randn(n, n, dtype=torch.float32, device="cuda")
+    Bs = torch.randn(n, n, dtype=torch.float32, device="cuda")
+    Cs = torch.empty_like(As)
+    ctx.sgemm_dev(As, Bs, Cs)
+    ms, _ = timed(torch, lambda: ctx.sgemm_dev(As, Bs, Cs), reps)
+    tf = 2.0 * n ** 3 / (ms * 1e-3) / 1e12
+    out["sgemm_8192"] = {"ms": ms, "tflops": tf, "frac_of_fp32_peak": tf / PEAKS["fp32"]}
+    del As, Bs, Cs
+    n = 16384
+    G = torch.empty((n, n), dtype=torch.float64, device="cuda")
+    ctx.fill_normal_dev(2026, 0, G)
+    S = torch.empty_like(G)
+    ctx.gemm_dev(G, G.t().contiguous(), S, alpha=1.0 / n)
+    S.diagonal().add_(1.0)
+    del G
+    L = torch.empty_like(S)
+    b = torch.empty((n, 1), dtype=torch.float64, device="cuda")
+    ctx.fill_normal_dev(2026, 1, b)
+    x = torch.empty_like(b)
+    st = torch.zeros(1, dtype=torch.int32, device="cuda")
+    t = []
+    for it in range(reps + 1):
+        L.copy_(S)
+        ms, _ = timed(torch, lambda: (ctx.potrf_dev(L, st), ctx.potrs_dev(L, b, x)))
+        if it:
+            t.append(ms)
+    ms = sum(t) / len(t)
+    flop = n ** 3 / 3.0 + 2.0 * n ** 2
+    eta = ctx.backward_error_dev(S, x, b)
+    out["cholesky_n16384"] = {"ms": ms, "tflops": flop / (ms * 1e-3) / 1e12,
+                              "frac_of_fp64_peak": flop / (ms * 1e-3) / 1e12 / PEAKS["fp64"],
+                              "backward_error": eta, "status": int(st.item())}
+    return out
+
+
 def load_traffic():
     """HBM bytes per launch by (kernel class, workload) from the committed
     counter summary (profiles/traffic_per_launch.json)."""
@@ -485,6 +550,7 @@ def main():
         if not args.no_cfg2:
             out["cfg2_n8192"] = cfg2(ctx, torch, 2)
             out["qr"] = qr_bench(ctx, torch)
+            out["next_rows"] = next_rows(ctx, torch, 2)
         if not args.no_batched:
             out["batched_cholesky"] = batched_cholesky(ctx, args.steps, args.warmup)
             out["batched_lu_ir"] = batched_lu_ir(ctx, args.steps, args.warmup)

@@ -164,6 +164,8 @@ _SIGNATURES = {
     "la_dqr_q_dev": [_vp, _i64, _i64, _vp, _i64, _vp, _vp, _i64],
     "la_dqrs_dev": [_vp, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _i64],
     "la_dinverse_dev": [_vp, _i64, _vp, _i64, _vp, _vp, _i64, _vp],
+    "la_sgemm_dev": [_vp, _i64, _i64, _i64, ctypes.c_float, _vp, _i64, _vp, _i64,
+                     ctypes.c_float, _vp, _i64],
     "la_autotune": [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p],
     "la_autotune_load": [_vp, ctypes.c_char_p],
 }
@@ -431,6 +433,19 @@ class Context:
         self._check(self.lib.la_dgetrs_dev(self.h, m, n, _ptr(LU), LU.stride(0), _ptr(piv),
                                            B.shape[1], _ptr(B), B.stride(0), _ptr(X),
                                            X.stride(0), _ptr(status)), "dgetrs_dev")
+
+    def inverse_dev(self, A, piv, X, info):
+        """la_dinverse_dev: A becomes its packed LU, X its inverse."""
+        n = A.shape[0]
+        self._check(self.lib.la_dinverse_dev(self.h, n, _ptr(A), A.stride(0), _ptr(piv), _ptr(X),
+                                             X.stride(0), _ptr(info)), "dinverse_dev")
+
+    def sgemm_dev(self, A, B, C, alpha=1.0, beta=0.0):
+        """la_sgemm_dev (Matrix<f32>::simd_mul's product on FP32 MFMA)."""
+        m, k = A.shape
+        n = B.shape[1]
+        self._check(self.lib.la_sgemm_dev(self.h, m, n, k, alpha, _ptr(A), A.stride(0), _ptr(B),
+                                          B.stride(0), beta, _ptr(C), C.stride(0)), "sgemm_dev")
 
     def potrf_dev(self, A, status):
         n = A.shape[0]
