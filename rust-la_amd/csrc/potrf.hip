@@ -5,15 +5,19 @@
 //   - a diagonal pivot d <= 0 -> None (a NaN pivot is NOT rejected: d <= 0 is
 //     false for NaN, so the factor carries the NaN on, as the reference does)
 //   - the result is the full n x n L with the strict upper triangle zeroed.
-// Per 64-column block: the diagonal block is factored in LDS by one
-// workgroup, the panel below it is solved against L11^T row by row, and the
-// trailing lower triangle gets a rank-64 update on FP64 MFMA (the GEMM
-// kernel with B = panel^T, tiles above the diagonal skipped).
+// Two-level blocking.  Per 64-column panel: the diagonal block is factored
+// in LDS by one workgroup and the panel below it is solved against L11^T row
+// by row; the rest of its 256-column outer block gets the panel's rank-64
+// update, and once the outer block is done the trailing lower triangle gets
+// one rank-256 update on FP64 MFMA (the GEMM kernel with B = panel^T, tiles
+// above the diagonal skipped).  A rank-64 update of the whole trailing
+// triangle per panel would read and write it four times as often.
 #include "la_common.hpp"
 
 namespace lahip {
 
 constexpr int PB = 64;
+constexpr int PB_OUT = 256;     // outer block: K of the trailing update
 
 // status: bit 1 = asymmetric pair seen.  Compares 64 x 64 tile (bi, bj),
 // bi > bj or bi == bj, against its mirror through LDS.
@@ -148,18 +152,33 @@ int potrf_blocked(la_ctx *ctx, int64_t n, double *A, int64_t lda, int32_t *d_sta
     hipLaunchKernelGGL(k_symcheck, dim3((unsigned)(nbt * (nbt + 1) / 2)), dim3(256), 0,
                        ctx->stream, (int)n, (const double *)A, lda, flag);
     LA_HIP_CHECK(ctx, hipGetLastError());
-    for (int64_t k = 0; k < n; k += PB) {
-        const int kb = (int)std::min<int64_t>(PB, n - k);
-        double *Akk = A + k * lda + k;
-        hipLaunchKernelGGL(k_potf2, dim3(1), dim3(256), 0, ctx->stream, Akk, lda, kb, flag);
-        const int64_t below = n - k - kb;
-        if (below > 0) {
+    for (int64_t K0 = 0; K0 < n; K0 += PB_OUT) {
+        const int64_t K1 = std::min<int64_t>(n, K0 + PB_OUT);
+        for (int64_t k = K0; k < K1; k += PB) {
+            const int kb = (int)std::min<int64_t>(PB, K1 - k);
+            double *Akk = A + k * lda + k;
+            hipLaunchKernelGGL(k_potf2, dim3(1), dim3(256), 0, ctx->stream, Akk, lda, kb, flag);
+            const int64_t below = n - k - kb;
+            if (below <= 0) continue;
             double *A21 = A + (k + kb) * lda + k;
             hipLaunchKernelGGL(k_trsm_right_lt, dim3((unsigned)ceil_div(below, 64)), dim3(256), 0,
                                ctx->stream, (const double *)Akk, lda, kb, A21, lda, (int)below);
             LA_HIP_CHECK(ctx, hipGetLastError());
-            int st = gemm_rm<double>(ctx, below, below, kb, -1.0, A21, lda, A21, lda, 1.0,
-                                     A + (k + kb) * lda + k + kb, lda, true, true);
+            // the rest of the outer block: columns [k + kb, K1), rows below
+            // them (the tiles above the block's diagonal skipped)
+            const int64_t cols = K1 - (k + kb);
+            if (cols > 0) {
+                int st = gemm_rm<double>(ctx, below, cols, kb, -1.0, A21, lda, A21, lda, 1.0,
+                                         A + (k + kb) * lda + k + kb, lda, true, true);
+                if (st) return st;
+            }
+        }
+        // the trailing lower triangle, rank K1 - K0
+        const int64_t rest = n - K1;
+        if (rest > 0) {
+            const double *L21 = A + K1 * lda + K0;
+            int st = gemm_rm<double>(ctx, rest, rest, K1 - K0, -1.0, L21, lda, L21, lda, 1.0,
+                                     A + K1 * lda + K1, lda, true, true);
             if (st) return st;
         }
     }
