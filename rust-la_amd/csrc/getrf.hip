@@ -1076,11 +1076,15 @@ static int launch_panel(la_ctx *ctx, T *Apanel, int64_t lda, int M, int NB, int 
     uint64_t *diagp = reinterpret_cast<uint64_t *>(pw.diagbuf);
     void *args[] = {&Apanel, &lda, &M, &NB, &R, &kglob, &d_ipiv, &candp, &diagp,
                     &swapmap, &d_status, &ticks, &linv};
-    if (mask_cus > 0)
-        LA_HIP_CHECK(ctx, hipLaunchKernel(fn, dim3(P), dim3(256), args, shmem, ctx->stream));
-    else
-        LA_HIP_CHECK(ctx, hipLaunchCooperativeKernel(fn, dim3(P), dim3(256), args,
-                                                     (unsigned)shmem, ctx->stream));
+    // An ordinary launch in both cases.  The grid is co-resident by
+    // construction (P <= min(256, CUs) at one workgroup per CU or, masked,
+    // P <= the masked CUs x occupancy; dispatch is balanced over XCDs and
+    // shader engines), and every wait in the kernel is bounded.  The
+    // cooperative-launch API is not used: on this ROCm one process at a time
+    // can hold its device-wide queue, and another process's cooperative
+    // launch then fails ("operation cannot be performed in the present
+    // state"), so two processes factoring on one GPU would break each other.
+    LA_HIP_CHECK(ctx, hipLaunchKernel(fn, dim3(P), dim3(256), args, shmem, ctx->stream));
     return LA_OK;
 }
 

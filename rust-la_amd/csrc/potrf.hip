@@ -50,41 +50,56 @@ void k_symcheck(int n, const double *A, int64_t lda, int32_t *flag)
 
 // Factor the kb x kb diagonal block at A (kb <= 64) in LDS.
 // status bit 2: non-positive pivot (d <= 0).
-__global__ __launch_bounds__(256)
+__device__ inline double potf2_bcast(double v, int lane)
+{
+    const long long bits = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)bits, lane);
+    const int hi = __builtin_amdgcn_readlane((int)(bits >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// The kb x kb diagonal block (kb <= 64), right-looking, by one wavefront:
+// lane r keeps row r in registers (fully unrolled, so every index is
+// static); per column t the pivot and each a[c][t] reach all lanes by
+// readlane.  Per element the same updates in the same order as a
+// workgroup-wide LDS version: a[r][t] /= sqrt(d_t), then
+// a[r][c] -= a[r][t] * a[c][t] for t < c <= r.  A pivot d <= 0 sets bit 2 of
+// the flag (NaN is not rejected and propagates, as in the reference).
+__global__ __launch_bounds__(64)
 void k_potf2(double *A, int64_t lda, int kb, int32_t *flag)
 {
-    __shared__ double a[64][65];
-    for (int e = threadIdx.x; e < 64 * 64; e += 256) {
-        int r = e >> 6, c = e & 63;
-        a[r][c] = (r < kb && c <= r) ? A[(int64_t)r * lda + c] : 0.0;
+    const int r = threadIdx.x;
+    double a[64];
+    const int rc = min(r, kb - 1);
+#pragma unroll
+    for (int c = 0; c < 64; ++c) {
+        const double v = A[(int64_t)rc * lda + min(c, kb - 1)];
+        a[c] = (r < kb && c <= r) ? v : 0.0;
     }
-    __syncthreads();
-    __shared__ int s_bad;
-    if (threadIdx.x == 0) s_bad = 0;
-    __syncthreads();
-    for (int t = 0; t < kb; ++t) {
-        if (threadIdx.x == 0) {
-            double d = a[t][t];
-            if (d <= 0.0) s_bad = 1;
-            a[t][t] = sqrt(d);
+    bool bad = false;
+#pragma unroll
+    for (int t = 0; t < 64; ++t) {
+        if (t < kb) {
+            const double d = potf2_bcast(a[t], t);
+            if (d <= 0.0) bad = true;
+            const double dt = sqrt(d);
+            if (r == t) a[t] = dt;
+            if (r > t) a[t] = a[t] / dt;
+#pragma unroll
+            for (int c = t + 1; c < 64; ++c) {
+                if (c < kb) {
+                    const double act = potf2_bcast(a[t], c);
+                    if (c <= r) a[c] -= a[t] * act;
+                }
+            }
         }
-        __syncthreads();
-        const double dt = a[t][t];
-        for (int r = t + 1 + threadIdx.x; r < kb; r += 256) a[r][t] = a[r][t] / dt;
-        __syncthreads();
-        // trailing lower update: a[r][c] -= a[r][t]*a[c][t], t < c <= r
-        const int m = kb - t - 1;
-        for (int e = threadIdx.x; e < m * m; e += 256) {
-            int r = t + 1 + e / m, c = t + 1 + e % m;
-            if (c <= r) a[r][c] -= a[r][t] * a[c][t];
-        }
-        __syncthreads();
     }
-    for (int e = threadIdx.x; e < kb * kb; e += 256) {
-        int r = e / kb, c = e % kb;
-        A[(int64_t)r * lda + c] = (c <= r) ? a[r][c] : 0.0;
+    if (r < kb) {
+#pragma unroll
+        for (int c = 0; c < 64; ++c)
+            if (c < kb) A[(int64_t)r * lda + c] = (c <= r) ? a[c] : 0.0;
     }
-    if (threadIdx.x == 0 && s_bad) atomicOr((int *)flag, 2);
+    if (r == 0 && bad) atomicOr((int *)flag, 2);
 }
 
 // Panel solve: rows of B (nrows x kb, row-major) <- B * L11^-T with L11 the
@@ -157,7 +172,7 @@ int potrf_blocked(la_ctx *ctx, int64_t n, double *A, int64_t lda, int32_t *d_sta
         for (int64_t k = K0; k < K1; k += PB) {
             const int kb = (int)std::min<int64_t>(PB, K1 - k);
             double *Akk = A + k * lda + k;
-            hipLaunchKernelGGL(k_potf2, dim3(1), dim3(256), 0, ctx->stream, Akk, lda, kb, flag);
+            hipLaunchKernelGGL(k_potf2, dim3(1), dim3(64), 0, ctx->stream, Akk, lda, kb, flag);
             const int64_t below = n - k - kb;
             if (below <= 0) continue;
             double *A21 = A + (k + kb) * lda + k;

@@ -689,8 +689,11 @@ int geqrf_device(la_ctx *ctx, int64_t m, int64_t n, double *A, int64_t lda, doub
             int want = cend < n ? 1 : 0;
             void *args[] = {&mi, &A, &lda, &k0i, (void *)&nb, &rdiag, &w.slots, &stp, &tk,
                             &w.V, &w.P, &want};
-            LA_HIP_CHECK(ctx, hipLaunchCooperativeKernel(fn, dim3(Pq), dim3(256), args,
-                                                         (unsigned)shmem, ctx->stream));
+            // an ordinary launch: at most 128 workgroups of one per CU are
+            // co-resident on the full device, every wait is bounded, and the
+            // cooperative-launch API would tie the device to this process
+            // (another process's cooperative launch then fails)
+            LA_HIP_CHECK(ctx, hipLaunchKernel(fn, dim3(Pq), dim3(256), args, shmem, ctx->stream));
             if (want) {
                 // the block's V^T V from the panel's per-workgroup partials, then T
                 hipLaunchKernelGGL(k_qr_vtc_reduce, dim3((unsigned)ceil_div(nb * nb, 256)),
