@@ -1016,9 +1016,9 @@ static int masked_panel_rpl(int M, int cus)
     return 0;
 }
 
-// mask_cus > 0: the panel runs as an ordinary launch on a stream whose CU
-// mask holds mask_cus CUs (masked_panel_rpl must have accepted M); 0: a
-// cooperative launch over the whole device.
+// mask_cus > 0: the panel runs on a stream whose CU mask holds mask_cus CUs
+// (masked_panel_rpl must have accepted M); 0: on the whole device, at most
+// one workgroup per CU.
 template <typename T>
 static int launch_panel(la_ctx *ctx, T *Apanel, int64_t lda, int M, int NB, int kglob,
                         int64_t *d_ipiv, PanelWS &pw, int64_t *swapmap, int32_t *d_status,
