@@ -48,8 +48,6 @@ void k_symcheck(int n, const double *A, int64_t lda, int32_t *flag)
     if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr((int *)flag, 1);
 }
 
-// Factor the kb x kb diagonal block at A (kb <= 64) in LDS.
-// status bit 2: non-positive pivot (d <= 0).
 __device__ inline double potf2_bcast(double v, int lane)
 {
     const long long bits = __double_as_longlong(v);
@@ -58,87 +56,123 @@ __device__ inline double potf2_bcast(double v, int lane)
     return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
-// The kb x kb diagonal block (kb <= 64), right-looking, by one wavefront:
-// lane r keeps row r in registers (fully unrolled, so every index is
-// static); per column t the pivot and each a[c][t] reach all lanes by
-// readlane.  Per element the same updates in the same order as a
-// workgroup-wide LDS version: a[r][t] /= sqrt(d_t), then
-// a[r][c] -= a[r][t] * a[c][t] for t < c <= r.  A pivot d <= 0 sets bit 2 of
-// the flag (NaN is not rejected and propagates, as in the reference).
+// The kb x kb diagonal block (kb <= 64) by one wavefront, lane r owning row
+// r, eight columns at a time.  Element (r, c) is
+//   l[r][c] = (a[r][c] - sum_{k<c} l[r][k] * l[c][k]) / sqrt(d_c)
+// with the sum taken as fma(-l[r][k], l[c][k], .) in k order and d_c the
+// diagonal element after its sum: the same operations in the same order as
+// a right-looking sweep.  For columns c0..c0+7 the terms k < c0 (finished
+// columns, LDS) go to eight independent register accumulators; the terms
+// inside the block follow column by column, l[c][k] taken from lane c by
+// readlane.  Eight independent chains keep the FP64 pipeline busy where one
+// dot product per column would wait out the latency of every fma.  A pivot
+// d <= 0 sets bit 2 of the flag (NaN is not rejected and propagates, as in
+// the reference).
 __global__ __launch_bounds__(64)
 void k_potf2(double *A, int64_t lda, int kb, int32_t *flag)
 {
+    __shared__ double a[64][65];
     const int r = threadIdx.x;
-    double a[64];
     const int rc = min(r, kb - 1);
+    {
+        double v[64];
 #pragma unroll
-    for (int c = 0; c < 64; ++c) {
-        const double v = A[(int64_t)rc * lda + min(c, kb - 1)];
-        a[c] = (r < kb && c <= r) ? v : 0.0;
+        for (int i = 0; i < 64; ++i) v[i] = A[(int64_t)min(i, kb - 1) * lda + rc];
+#pragma unroll
+        for (int i = 0; i < 64; ++i) a[i][r] = (i < kb && r <= i) ? v[i] : 0.0;
     }
+    __syncthreads();
     bool bad = false;
+    for (int c0 = 0; c0 < kb; c0 += 8) {
+        double s[8];
 #pragma unroll
-    for (int t = 0; t < 64; ++t) {
-        if (t < kb) {
-            const double d = potf2_bcast(a[t], t);
-            if (d <= 0.0) bad = true;
-            const double dt = sqrt(d);
-            if (r == t) a[t] = dt;
-            if (r > t) a[t] = a[t] / dt;
+        for (int j = 0; j < 8; ++j) s[j] = a[r][c0 + j];
+        for (int k = 0; k < c0; ++k) {
+            const double lrk = a[r][k];
 #pragma unroll
-            for (int c = t + 1; c < 64; ++c) {
-                if (c < kb) {
-                    const double act = potf2_bcast(a[t], c);
-                    if (c <= r) a[c] -= a[t] * act;
-                }
+            for (int j = 0; j < 8; ++j) s[j] = fma(-lrk, a[c0 + j][k], s[j]);
+        }
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const int c = c0 + t;
+            if (c < kb) {
+                const double d = potf2_bcast(s[t], c);
+                if (d <= 0.0) bad = true;
+                const double dt = sqrt(d);
+                s[t] = (r == c) ? dt : s[t] / dt;
+#pragma unroll
+                for (int j = t + 1; j < 8; ++j) s[j] = fma(-s[t], potf2_bcast(s[t], c0 + j), s[j]);
             }
         }
+        // entries right of the diagonal (lanes r < c) are scratch: never
+        // read as l[.][.] and not stored
+#pragma unroll
+        for (int j = 0; j < 8; ++j) a[r][c0 + j] = s[j];
+        __syncthreads();
     }
     if (r < kb) {
-#pragma unroll
-        for (int c = 0; c < 64; ++c)
-            if (c < kb) A[(int64_t)r * lda + c] = (c <= r) ? a[c] : 0.0;
+        for (int i = 0; i < kb; ++i) A[(int64_t)i * lda + r] = (r <= i) ? a[i][r] : 0.0;
     }
     if (r == 0 && bad) atomicOr((int *)flag, 2);
 }
 
 // Panel solve: rows of B (nrows x kb, row-major) <- B * L11^-T with L11 the
-// kb x kb lower factor at Lm.  A workgroup stages 64 rows of B in LDS and
-// sweeps the columns: x_t = b_t / L[t][t], then b_c -= x_t * L[c][t] for c > t.
-__global__ __launch_bounds__(256)
+// kb x kb lower factor at Lm.  Rows are independent: a one-wavefront
+// workgroup stages 64 rows and L11 (transposed, Lt[k][c] = L[c][k]) in LDS
+// and lane r solves its own row,
+//   x_c = (b_c - sum_{k<c} x_k * L[c][k]) / L[c][c]
+// with the sum taken as fma(-x_k, L[c][k], .) in k order (per element the
+// operations of the right-looking sweep).  Eight columns at a time: the
+// terms of finished columns go to eight independent register accumulators,
+// then the block is solved column by column.  No barrier inside the solve.
+__global__ __launch_bounds__(64)
 void k_trsm_right_lt(const double *Lm, int64_t ldl, int kb, double *B, int64_t ldb, int nrows)
 {
-    __shared__ double Ls[64][65];
+    __shared__ double Lt[64][65];
     __shared__ double Bs[64][65];
-    const int tid = threadIdx.x;
+    const int r = threadIdx.x;
     const int row0 = blockIdx.x * 64;
-    // unconditional loads from clamped positions (issued together), then
-    // the out-of-range elements are zeroed
-    double lv[16], bv[16];
+    const int rc = min(r, kb - 1);
+    {
+        double v[64];
 #pragma unroll
-    for (int k = 0; k < 16; ++k) {
-        const int e = tid + 256 * k, r = e >> 6, c = e & 63;
-        lv[k] = Lm[(int64_t)min(r, kb - 1) * ldl + min(c, kb - 1)];
-        bv[k] = B[(int64_t)min(row0 + r, nrows - 1) * ldb + min(c, kb - 1)];
+        for (int i = 0; i < 64; ++i) v[i] = Lm[(int64_t)min(i, kb - 1) * ldl + rc];
+#pragma unroll
+        for (int i = 0; i < 64; ++i) Lt[r][i] = (i < kb && r <= i) ? v[i] : 0.0;
     }
+    {
+        double v[64];
 #pragma unroll
-    for (int k = 0; k < 16; ++k) {
-        const int e = tid + 256 * k, r = e >> 6, c = e & 63;
-        Ls[r][c] = (r < kb && c <= r) ? lv[k] : 0.0;
-        Bs[r][c] = (row0 + r < nrows && c < kb) ? bv[k] : 0.0;
+        for (int i = 0; i < 64; ++i) v[i] = B[(int64_t)min(row0 + i, nrows - 1) * ldb + rc];
+#pragma unroll
+        for (int i = 0; i < 64; ++i) Bs[i][r] = (row0 + i < nrows && r < kb) ? v[i] : 0.0;
     }
     __syncthreads();
-    const int r = tid & 63, cq = tid >> 6;
-    for (int t = 0; t < kb; ++t) {
-        if (cq == 0) Bs[r][t] = Bs[r][t] / Ls[t][t];
-        __syncthreads();
-        const double xt = Bs[r][t];
-        for (int c = t + 1 + ((cq - (t + 1)) & 3); c < kb; c += 4) Bs[r][c] -= xt * Ls[c][t];
-        __syncthreads();
+    for (int c0 = 0; c0 < kb; c0 += 8) {
+        double s[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) s[j] = Bs[r][c0 + j];
+        for (int k = 0; k < c0; ++k) {
+            const double xk = Bs[r][k];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) s[j] = fma(-xk, Lt[k][c0 + j], s[j]);
+        }
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const int c = c0 + t;
+            if (c < kb) {
+                s[t] = s[t] / Lt[c][c];
+#pragma unroll
+                for (int j = t + 1; j < 8; ++j) s[j] = fma(-s[t], Lt[c][c0 + j], s[j]);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) Bs[r][c0 + j] = s[j];
     }
-    for (int e = tid; e < 64 * 64; e += 256) {
-        int rr = e >> 6, c = e & 63;
-        if (row0 + rr < nrows && c < kb) B[(int64_t)(row0 + rr) * ldb + c] = Bs[rr][c];
+    __syncthreads();
+    if (r < kb) {
+        const int nr = min(64, nrows - row0);
+        for (int i = 0; i < nr; ++i) B[(int64_t)(row0 + i) * ldb + r] = Bs[i][r];
     }
 }
 
@@ -176,7 +210,7 @@ int potrf_blocked(la_ctx *ctx, int64_t n, double *A, int64_t lda, int32_t *d_sta
             const int64_t below = n - k - kb;
             if (below <= 0) continue;
             double *A21 = A + (k + kb) * lda + k;
-            hipLaunchKernelGGL(k_trsm_right_lt, dim3((unsigned)ceil_div(below, 64)), dim3(256), 0,
+            hipLaunchKernelGGL(k_trsm_right_lt, dim3((unsigned)ceil_div(below, 64)), dim3(64), 0,
                                ctx->stream, (const double *)Akk, lda, kb, A21, lda, (int)below);
             LA_HIP_CHECK(ctx, hipGetLastError());
             // the rest of the outer block: columns [k + kb, K1), rows below
