@@ -41,10 +41,14 @@ constexpr int GBM = 128, GBN = 128, GBK = 16;
 // XCD-aware tile order: consecutive workgroup ids are dealt round-robin to
 // the 8 XCDs, so remap ids so that each XCD receives a contiguous range of
 // logical tiles, then walk tiles in groups of 8 tile-rows for L2 reuse.
+// Not for a lower-triangular update (remap = false): contiguous ranges would
+// hand the XCDs owning the top tile-rows mostly empty tiles; dealt
+// round-robin, XCD x takes tile-row 8g + x of every group, so each XCD keeps
+// its row of A in its L2 and the XCDs get nearly equal work.
 __device__ inline void tile_coords(int bid, int ntiles, int tiles_m, int tiles_n,
-                                   int &tm, int &tn) {
+                                   int &tm, int &tn, bool remap = true) {
     int logical = bid;
-    if ((ntiles & 7) == 0) {
+    if (remap && (ntiles & 7) == 0) {
         int per = ntiles >> 3;
         logical = (bid & 7) * per + (bid >> 3);
     }
@@ -69,7 +73,7 @@ void k_gemm_rm(int M, int N, int K, T alpha, const T *__restrict__ A, int64_t ld
     __shared__ T Bs[2][GBK][LDB_S];
 
     int tm, tn;
-    tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, tm, tn);
+    tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, tm, tn, !lower_only);
     const int m0 = tm * GBM, n0 = tn * GBN;
     if (lower_only && n0 > m0 + GBM - 1) return;
 
@@ -186,11 +190,13 @@ void k_gemm_rm(int M, int N, int K, T alpha, const T *__restrict__ A, int64_t ld
         __syncthreads();
     }
 
-    // epilogue.  Interior tiles read C (beta != 0) sixteen elements at a
-    // time with no per-element condition, so those loads issue together;
-    // edge tiles and the lower-only (SYRK) case keep per-element guards.
-    // Both forms evaluate alpha * acc + beta * c in the same order.
-    if (full_mn && !lower_only) {
+    // epilogue.  Interior tiles (with lower_only: those entirely below the
+    // diagonal, last column <= first row) read C (beta != 0) sixteen
+    // elements at a time with no per-element condition, so those loads issue
+    // together; edge tiles and the tiles the diagonal crosses keep
+    // per-element guards.  Both forms evaluate alpha * acc + beta * c in the
+    // same order.
+    if (full_mn && (!lower_only || n0 + GBN - 1 <= m0)) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             T cv[4][4];
