@@ -16,8 +16,9 @@
 
 namespace lahip {
 
-constexpr int PB = 64;
-constexpr int PB_OUT = 256;     // outer block: K of the trailing update
+constexpr int PB = 64;          // panel: factored and solved by latency-bound kernels
+constexpr int PB_MID = 256;     // middle block: rank of the updates inside an outer block
+constexpr int PB_OUT = 1024;    // outer block: rank of the trailing update
 
 // status: bit 1 = asymmetric pair seen.  Compares 64 x 64 tile (bi, bj),
 // bi > bj or bi == bj, against its mirror through LDS.
@@ -201,35 +202,40 @@ int potrf_blocked(la_ctx *ctx, int64_t n, double *A, int64_t lda, int32_t *d_sta
     hipLaunchKernelGGL(k_symcheck, dim3((unsigned)(nbt * (nbt + 1) / 2)), dim3(256), 0,
                        ctx->stream, (int)n, (const double *)A, lda, flag);
     LA_HIP_CHECK(ctx, hipGetLastError());
+    // lower-triangular update of the block whose top-left corner (r0, r0)
+    // lies on the diagonal: rows r0..n-1, columns r0..c1-1, minus the
+    // product of the factored columns k0..k1-1 of those rows with the
+    // transpose of their first c1 - r0 rows
+    auto syrk = [&](int64_t r0, int64_t c1, int64_t k0, int64_t k1) -> int {
+        const int64_t rows = n - r0, cols = c1 - r0;
+        if (rows <= 0 || cols <= 0) return LA_OK;
+        const double *P = A + r0 * lda + k0;
+        return gemm_rm<double>(ctx, rows, cols, k1 - k0, -1.0, P, lda, P, lda, 1.0,
+                               A + r0 * lda + r0, lda, true, true);
+    };
     for (int64_t K0 = 0; K0 < n; K0 += PB_OUT) {
         const int64_t K1 = std::min<int64_t>(n, K0 + PB_OUT);
-        for (int64_t k = K0; k < K1; k += PB) {
-            const int kb = (int)std::min<int64_t>(PB, K1 - k);
-            double *Akk = A + k * lda + k;
-            hipLaunchKernelGGL(k_potf2, dim3(1), dim3(64), 0, ctx->stream, Akk, lda, kb, flag);
-            const int64_t below = n - k - kb;
-            if (below <= 0) continue;
-            double *A21 = A + (k + kb) * lda + k;
-            hipLaunchKernelGGL(k_trsm_right_lt, dim3((unsigned)ceil_div(below, 64)), dim3(64), 0,
-                               ctx->stream, (const double *)Akk, lda, kb, A21, lda, (int)below);
-            LA_HIP_CHECK(ctx, hipGetLastError());
-            // the rest of the outer block: columns [k + kb, K1), rows below
-            // them (the tiles above the block's diagonal skipped)
-            const int64_t cols = K1 - (k + kb);
-            if (cols > 0) {
-                int st = gemm_rm<double>(ctx, below, cols, kb, -1.0, A21, lda, A21, lda, 1.0,
-                                         A + (k + kb) * lda + k + kb, lda, true, true);
+        for (int64_t M0 = K0; M0 < K1; M0 += PB_MID) {
+            const int64_t M1 = std::min<int64_t>(K1, M0 + PB_MID);
+            for (int64_t k = M0; k < M1; k += PB) {
+                const int kb = (int)std::min<int64_t>(PB, M1 - k);
+                double *Akk = A + k * lda + k;
+                hipLaunchKernelGGL(k_potf2, dim3(1), dim3(64), 0, ctx->stream, Akk, lda, kb,
+                                   flag);
+                const int64_t below = n - k - kb;
+                if (below <= 0) continue;
+                hipLaunchKernelGGL(k_trsm_right_lt, dim3((unsigned)ceil_div(below, 64)), dim3(64),
+                                   0, ctx->stream, (const double *)Akk, lda, kb,
+                                   A + (k + kb) * lda + k, lda, (int)below);
+                LA_HIP_CHECK(ctx, hipGetLastError());
+                int st = syrk(k + kb, M1, k, k + kb);          // rest of the middle block
                 if (st) return st;
             }
-        }
-        // the trailing lower triangle, rank K1 - K0
-        const int64_t rest = n - K1;
-        if (rest > 0) {
-            const double *L21 = A + K1 * lda + K0;
-            int st = gemm_rm<double>(ctx, rest, rest, K1 - K0, -1.0, L21, lda, L21, lda, 1.0,
-                                     A + K1 * lda + K1, lda, true, true);
+            int st = syrk(M1, K1, M0, M1);                      // rest of the outer block
             if (st) return st;
         }
+        int st = syrk(K1, n, K0, K1);                           // the trailing triangle
+        if (st) return st;
     }
     hipLaunchKernelGGL(k_zero_upper, dim3((unsigned)ceil_div(n * n, 256)), dim3(256), 0,
                        ctx->stream, n, A, lda);
