@@ -245,7 +245,10 @@ def test_cholesky_kats(ctx):
         ctx.chol_solve(l, m([1.0], [2.0], [3.0], [4.0]))
 
 
-@pytest.mark.parametrize("n", [1, 7, 64, 65, 300, 777])
+# 777: panels, middle-block updates and a ragged last panel inside one outer
+# block; 1300 and 2300: the rank-1024 trailing update, then a ragged outer
+# block
+@pytest.mark.parametrize("n", [1, 7, 64, 65, 300, 777, 1300, 2300])
 def test_cholesky_vs_oracle(ctx, n):
     a = oracle.spd_matrix(n, 17, 3)
     b = oracle.normal(17, 4, (n, 2))
