@@ -5,13 +5,14 @@
 //   - a diagonal pivot d <= 0 -> None (a NaN pivot is NOT rejected: d <= 0 is
 //     false for NaN, so the factor carries the NaN on, as the reference does)
 //   - the result is the full n x n L with the strict upper triangle zeroed.
-// Two-level blocking.  Per 64-column panel: the diagonal block is factored
-// in LDS by one workgroup and the panel below it is solved against L11^T row
-// by row; the rest of its 256-column outer block gets the panel's rank-64
-// update, and once the outer block is done the trailing lower triangle gets
-// one rank-256 update on FP64 MFMA (the GEMM kernel with B = panel^T, tiles
-// above the diagonal skipped).  A rank-64 update of the whole trailing
-// triangle per panel would read and write it four times as often.
+// Three-level blocking.  Per 64-column panel: the diagonal block is factored
+// by one wavefront and the panel below it is solved against L11^T one row
+// per lane; the rest of its 256-column middle block gets the panel's
+// rank-64 update.  Once a middle block is done, the rest of its 1024-column
+// outer block gets one rank-256 update, and once an outer block is done the
+// trailing lower triangle gets one rank-1024 update.  Every update is the
+// FP64 MFMA GEMM with B = panel^T and the tiles above the diagonal skipped;
+// wider ranks amortise reading and writing the trailing triangle.
 #include "la_common.hpp"
 
 namespace lahip {
