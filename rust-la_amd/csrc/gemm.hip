@@ -41,14 +41,10 @@ constexpr int GBM = 128, GBN = 128, GBK = 16;
 // XCD-aware tile order: consecutive workgroup ids are dealt round-robin to
 // the 8 XCDs, so remap ids so that each XCD receives a contiguous range of
 // logical tiles, then walk tiles in groups of 8 tile-rows for L2 reuse.
-// Not for a lower-triangular update (remap = false): contiguous ranges would
-// hand the XCDs owning the top tile-rows mostly empty tiles; dealt
-// round-robin, XCD x takes tile-row 8g + x of every group, so each XCD keeps
-// its row of A in its L2 and the XCDs get nearly equal work.
 __device__ inline void tile_coords(int bid, int ntiles, int tiles_m, int tiles_n,
-                                   int &tm, int &tn, bool remap = true) {
+                                   int &tm, int &tn) {
     int logical = bid;
-    if (remap && (ntiles & 7) == 0) {
+    if ((ntiles & 7) == 0) {
         int per = ntiles >> 3;
         logical = (bid & 7) * per + (bid >> 3);
     }
@@ -59,6 +55,50 @@ __device__ inline void tile_coords(int bid, int ntiles, int tiles_m, int tiles_n
     int in_group = logical - group * G * tiles_n;
     tm = first_m + in_group % gsize;
     tn = in_group / gsize;
+}
+
+// Tiles on or below the diagonal (tn <= tm, tn < tiles_n) of a
+// lower-triangular update: sum over tm of min(tm + 1, tiles_n).
+__host__ __device__ inline int lower_tile_count(int tiles_m, int tiles_n) {
+    return tiles_m <= tiles_n ? tiles_m * (tiles_m + 1) / 2
+                              : tiles_n * (tiles_n + 1) / 2 + (tiles_m - tiles_n) * tiles_n;
+}
+
+// Logical index -> tile of a lower-triangular update, in the order of
+// tile_coords restricted to the lower tiles: groups of 8 tile-rows, inside
+// a group by column, then by row.  A group holds every row of the columns
+// left of its diagonal and a shrinking run of rows for the columns it
+// crosses.  The grid enumerates only these tiles, so with the XCD range
+// remap every XCD receives an equal, contiguous share of real work.
+__device__ inline void lower_tile_coords(int logical, int tiles_m, int tiles_n, int &tm,
+                                         int &tn) {
+    const int G = 8;
+    for (int first_m = 0; first_m < tiles_m; first_m += G) {
+        const int gsize = min(G, tiles_m - first_m);
+        const int full_cols = min(first_m, tiles_n);
+        const int tri = max(0, min(gsize, tiles_n - first_m));
+        const int cnt = full_cols * gsize + tri * gsize - tri * (tri - 1) / 2;
+        if (logical < cnt) {
+            if (logical < full_cols * gsize) {
+                tn = logical / gsize;
+                tm = first_m + logical % gsize;
+                return;
+            }
+            int o = logical - full_cols * gsize;
+            for (int j = 0; j < tri; ++j) {
+                const int c = gsize - j;          // rows first_m + j .. first_m + gsize - 1
+                if (o < c) {
+                    tn = first_m + j;
+                    tm = first_m + j + o;
+                    return;
+                }
+                o -= c;
+            }
+        }
+        logical -= cnt;
+    }
+    tm = tiles_m;
+    tn = 0;
 }
 
 template <typename T, bool TRANS_B>
@@ -73,9 +113,17 @@ void k_gemm_rm(int M, int N, int K, T alpha, const T *__restrict__ A, int64_t ld
     __shared__ T Bs[2][GBK][LDB_S];
 
     int tm, tn;
-    tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, tm, tn, !lower_only);
+    if (lower_only) {
+        // the grid is the lower tiles rounded up to a multiple of 8
+        const int per = gridDim.x >> 3;
+        const int logical = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+        if (logical >= lower_tile_count(tiles_m, tiles_n)) return;
+        lower_tile_coords(logical, tiles_m, tiles_n, tm, tn);
+    } else {
+        tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, tm, tn);
+    }
     const int m0 = tm * GBM, n0 = tn * GBN;
-    if (lower_only && n0 > m0 + GBM - 1) return;
+    if (tm >= tiles_m || (lower_only && n0 > m0 + GBM - 1)) return;
 
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
@@ -251,8 +299,10 @@ int gemm_rm(la_ctx *ctx, int64_t M, int64_t N, int64_t K, T alpha,
         return set_error(ctx, LA_ERR_ARG, "gemm: extent exceeds int32");
     const int tiles_m = (int)ceil_div(M, GBM), tiles_n = (int)ceil_div(N, GBN);
     const int64_t ntiles = (int64_t)tiles_m * tiles_n;
-    if (ntiles > INT32_MAX) return set_error(ctx, LA_ERR_ARG, "gemm: too many tiles");
-    dim3 grid((unsigned)ntiles), block(256);
+    if (ntiles > INT32_MAX / 2) return set_error(ctx, LA_ERR_ARG, "gemm: too many tiles");
+    const int64_t nlaunch = lower_only ? (int64_t)(lower_tile_count(tiles_m, tiles_n) + 7) / 8 * 8
+                                       : ntiles;
+    dim3 grid((unsigned)nlaunch), block(256);
     const double flop = (lower_only ? 1.0 : 2.0) * (double)M * (double)N * (double)K;
     const double bytes = (double)sizeof(T) * ((double)M * K + (double)K * N +
                                               (beta != T(0) ? 2.0 : 1.0) * (double)M * N);
