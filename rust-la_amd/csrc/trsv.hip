@@ -391,9 +391,11 @@ int trsv_blocked(la_ctx *ctx, char uplo, bool trans, bool unit, int64_t n, const
 // ------------------------------------------- many right-hand sides (TRSM)
 // For nrhs > NR the substitution becomes a blocked TRSM on the FP64 MFMA
 // GEMM: the 64 x 64 diagonal blocks are inverted once (one workgroup each,
-// in parallel), then block by block in dependency order
-//   Y = inv(T_bb) X_b;   X_rest -= T_rest,b Y;   X_b = Y
-// (right-looking: the update of every later block is one GEMM of K = 64).
+// in parallel), then, in dependency order, per super-block of four blocks
+//   per block b:  Y = inv(T_bb) X_b;  X_(rest of super-block) -= T_.,b Y;  X_b = Y
+//   then          X_(rest of the triangle) -= T_.,super-block X_super-block
+// (right-looking: the update of the remaining rows is one GEMM of K = 256,
+// which runs the MFMA kernel far closer to its peak than four of K = 64).
 template <int MODE, typename TF>
 __global__ __launch_bounds__(256)
 void k_diag_inverses(int n, const TF *__restrict__ Tm, int64_t ldt, double *Dinv)
@@ -450,25 +452,44 @@ int trsm_blocked(la_ctx *ctx, char uplo, int64_t n, const double *Tm, int64_t ld
         hipLaunchKernelGGL((k_diag_inverses<MODE_LU_BWD, double>), dim3((unsigned)nblocks),
                            dim3(256), 0, ctx->stream, (int)n, Tm, ldt, Dinv);
     LA_HIP_CHECK(ctx, hipGetLastError());
-    for (int64_t s = 0; s < nblocks; ++s) {
-        const int64_t b = lower ? s : nblocks - 1 - s;
-        const int64_t r0 = b * TB, rb = std::min<int64_t>(TB, n - r0);
-        double *Xb = X + r0 * ldx;
-        if ((st = gemm_rm<double>(ctx, rb, nrhs, rb, 1.0, Dinv + (size_t)b * TB * TB, TB, Xb, ldx,
-                                  0.0, Y, nrhs, false, false)))
-            return st;
-        if (lower && r0 + rb < n) {
-            if ((st = gemm_rm<double>(ctx, n - r0 - rb, nrhs, rb, -1.0, Tm + (r0 + rb) * ldt + r0,
-                                      ldt, Y, nrhs, 1.0, X + (r0 + rb) * ldx, ldx, false, false)))
+    constexpr int64_t SBB = 4;                 // blocks per super-block
+    const int64_t nsuper = ceil_div(nblocks, SBB);
+    for (int64_t ss = 0; ss < nsuper; ++ss) {
+        const int64_t sb = lower ? ss : nsuper - 1 - ss;
+        const int64_t b_lo = sb * SBB, b_hi = std::min(nblocks, b_lo + SBB);
+        const int64_t R0 = b_lo * TB, R1 = std::min<int64_t>(n, b_hi * TB);
+        for (int64_t s = 0; s < b_hi - b_lo; ++s) {
+            const int64_t b = lower ? b_lo + s : b_hi - 1 - s;
+            const int64_t r0 = b * TB, rb = std::min<int64_t>(TB, n - r0);
+            double *Xb = X + r0 * ldx;
+            if ((st = gemm_rm<double>(ctx, rb, nrhs, rb, 1.0, Dinv + (size_t)b * TB * TB, TB, Xb,
+                                      ldx, 0.0, Y, nrhs, false, false)))
                 return st;
-        } else if (!lower && r0 > 0) {
-            if ((st = gemm_rm<double>(ctx, r0, nrhs, rb, -1.0, Tm + r0, ldt, Y, nrhs, 1.0, X, ldx,
-                                      false, false)))
+            // the rows of this super-block still to be solved
+            if (lower && r0 + rb < R1) {
+                if ((st = gemm_rm<double>(ctx, R1 - r0 - rb, nrhs, rb, -1.0,
+                                          Tm + (r0 + rb) * ldt + r0, ldt, Y, nrhs, 1.0,
+                                          X + (r0 + rb) * ldx, ldx, false, false)))
+                    return st;
+            } else if (!lower && r0 > R0) {
+                if ((st = gemm_rm<double>(ctx, r0 - R0, nrhs, rb, -1.0, Tm + R0 * ldt + r0, ldt, Y,
+                                          nrhs, 1.0, X + R0 * ldx, ldx, false, false)))
+                    return st;
+            }
+            hipLaunchKernelGGL(k_copy_rows, dim3((unsigned)ceil_div(rb * nrhs, 256)), dim3(256), 0,
+                               ctx->stream, rb, nrhs, (const double *)Y, nrhs, Xb, ldx);
+            LA_HIP_CHECK(ctx, hipGetLastError());
+        }
+        // every row outside the super-block, with the super-block's rows of X
+        if (lower && R1 < n) {
+            if ((st = gemm_rm<double>(ctx, n - R1, nrhs, R1 - R0, -1.0, Tm + R1 * ldt + R0, ldt,
+                                      X + R0 * ldx, ldx, 1.0, X + R1 * ldx, ldx, false, false)))
+                return st;
+        } else if (!lower && R0 > 0) {
+            if ((st = gemm_rm<double>(ctx, R0, nrhs, R1 - R0, -1.0, Tm + R0, ldt, X + R0 * ldx, ldx,
+                                      1.0, X, ldx, false, false)))
                 return st;
         }
-        hipLaunchKernelGGL(k_copy_rows, dim3((unsigned)ceil_div(rb * nrhs, 256)), dim3(256), 0,
-                           ctx->stream, rb, nrhs, (const double *)Y, nrhs, Xb, ldx);
-        LA_HIP_CHECK(ctx, hipGetLastError());
     }
     return LA_OK;
 }
