@@ -391,11 +391,11 @@ int trsv_blocked(la_ctx *ctx, char uplo, bool trans, bool unit, int64_t n, const
 // ------------------------------------------- many right-hand sides (TRSM)
 // For nrhs > NR the substitution becomes a blocked TRSM on the FP64 MFMA
 // GEMM: the 64 x 64 diagonal blocks are inverted once (one workgroup each,
-// in parallel), then, in dependency order, per super-block of four blocks
+// in parallel), then, in dependency order, per super-block of eight blocks
 //   per block b:  Y = inv(T_bb) X_b;  X_(rest of super-block) -= T_.,b Y;  X_b = Y
 //   then          X_(rest of the triangle) -= T_.,super-block X_super-block
-// (right-looking: the update of the remaining rows is one GEMM of K = 256,
-// which runs the MFMA kernel far closer to its peak than four of K = 64).
+// (right-looking: the update of the remaining rows is one GEMM of K = 512,
+// which runs the MFMA kernel far closer to its peak than eight of K = 64).
 template <int MODE, typename TF>
 __global__ __launch_bounds__(256)
 void k_diag_inverses(int n, const TF *__restrict__ Tm, int64_t ldt, double *Dinv)
@@ -452,7 +452,7 @@ int trsm_blocked(la_ctx *ctx, char uplo, int64_t n, const double *Tm, int64_t ld
         hipLaunchKernelGGL((k_diag_inverses<MODE_LU_BWD, double>), dim3((unsigned)nblocks),
                            dim3(256), 0, ctx->stream, (int)n, Tm, ldt, Dinv);
     LA_HIP_CHECK(ctx, hipGetLastError());
-    constexpr int64_t SBB = 4;                 // blocks per super-block
+    constexpr int64_t SBB = 8;                 // blocks per super-block
     const int64_t nsuper = ceil_div(nblocks, SBB);
     for (int64_t ss = 0; ss < nsuper; ++ss) {
         const int64_t sb = lower ? ss : nsuper - 1 - ss;
