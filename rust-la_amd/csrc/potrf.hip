@@ -50,6 +50,7 @@ void k_symcheck(int n, const double *A, int64_t lda, int32_t *flag)
     if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr((int *)flag, 1);
 }
 
+// v of lane `lane` (a wave-uniform index) to every lane of the wavefront
 __device__ inline double potf2_bcast(double v, int lane)
 {
     const long long bits = __double_as_longlong(v);
