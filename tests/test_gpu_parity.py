@@ -307,19 +307,25 @@ def test_matrix_solve_wide_status_order(ctx):
 
 
 # ------------------------------------------------ inverse (many RHS, TRSM)
-@pytest.mark.parametrize("n", [512, 1024])
+# 512: one super-block of eight 64-row blocks; 1024: two, with the update
+# of the rest of each triangle; 1100: a ragged last block and super-block
+@pytest.mark.parametrize("n", [512, 1024, 1100])
 def test_inverse_vs_oracle(ctx, n):
     """Matrix::inverse = LUDecomposition::new(a).solve(id) (mod.rs:1034-1037):
-    an n-column solve, which runs as the blocked MFMA TRSM; the result is
-    held to the oracle's inverse at 1e-12 normwise and A * inv(A) = I."""
+    an n-column solve, which runs as the blocked MFMA TRSM.  Two backward
+    stable inverses differ by a few u * cond(A) normwise (the oracle and
+    LAPACK's inverse differ by 1.4 u * cond(A) at n = 1100), so the result is
+    held to the oracle's inverse at 4 u * cond(A), and A * inv(A) = I."""
     a = oracle.uniform(77, n, (n, n))
     inv = ctx.inverse(a)
     assert np.array_equal(inv, ctx.solve(a, np.eye(n)))
     lu_o, piv_o, _ = oracle.lu_new(a)
     inv_o = oracle.lu_solve(lu_o, piv_o, np.eye(n))
     err = rel_fro(inv, inv_o)
-    print(f"inverse n={n}: rel. difference from the oracle {err:.2e}", flush=True)
-    assert err <= 1e-12
+    bound = 4 * np.finfo(np.float64).eps / 2 * np.linalg.cond(a)
+    print(f"inverse n={n}: rel. difference from the oracle {err:.2e} (bound {bound:.2e})",
+          flush=True)
+    assert err <= bound
     resid = np.abs(a @ inv - np.eye(n)).max()
     assert resid <= n * 1e-13
 
