@@ -35,10 +35,20 @@ void k_symcheck(int n, const double *A, int64_t lda, int32_t *flag)
     while ((int64_t)(bi + 1) * (bi + 2) / 2 <= idx) ++bi;
     const int bj = (int)(idx - (int64_t)bi * (bi + 1) / 2);
     const int r0 = bi * 64, c0 = bj * 64;
-    for (int e = threadIdx.x; e < 64 * 64; e += 256) {
-        int r = e >> 6, c = e & 63;
-        t1[r][c] = (r0 + r < n && c0 + c < n) ? A[(int64_t)(r0 + r) * lda + c0 + c] : 0.0;
-        t2[r][c] = (c0 + r < n && r0 + c < n) ? A[(int64_t)(c0 + r) * lda + r0 + c] : 0.0;
+    // both tiles' 32 loads per thread from clamped (in-bounds) positions,
+    // issued together; positions outside the matrix are never compared
+    double v1[16], v2[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const int e = threadIdx.x + 256 * k, r = e >> 6, c = e & 63;
+        v1[k] = A[(int64_t)min(r0 + r, n - 1) * lda + min(c0 + c, n - 1)];
+        v2[k] = A[(int64_t)min(c0 + r, n - 1) * lda + min(r0 + c, n - 1)];
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const int e = threadIdx.x + 256 * k, r = e >> 6, c = e & 63;
+        t1[r][c] = v1[k];
+        t2[r][c] = v2[k];
     }
     __syncthreads();
     int bad = 0;
