@@ -282,6 +282,48 @@ __device__ unsigned long long g_panel_phase[8];
 #define PHASE_MARK(k) do {} while (0)
 #endif
 
+// The rows x NB block at src (row stride lds) into LDS at dst (row stride
+// ldb), and back: sixteen loads per thread are issued before any is
+// stored, so the block costs one memory round trip per sixteen elements a
+// thread, not one per element.
+template <typename T>
+__device__ inline void stage_block(T *dst, int ldb, const T *src, int64_t lds, int rows, int NB)
+{
+    const int total = rows * NB, nt = (int)blockDim.x;
+    for (int e0 = (int)threadIdx.x; e0 < total; e0 += 16 * nt) {
+        T v[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int e = min(e0 + k * nt, total - 1), r = e / NB, c = e - r * NB;
+            v[k] = src[(int64_t)r * lds + c];
+        }
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int e = e0 + k * nt, r = e / NB, c = e - r * NB;
+            if (e < total) dst[r * ldb + c] = v[k];
+        }
+    }
+}
+
+template <typename T>
+__device__ inline void unstage_block(T *dst, int64_t ldd, const T *src, int ldb, int rows, int NB)
+{
+    const int total = rows * NB, nt = (int)blockDim.x;
+    for (int e0 = (int)threadIdx.x; e0 < total; e0 += 16 * nt) {
+        T v[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int e = min(e0 + k * nt, total - 1), r = e / NB, c = e - r * NB;
+            v[k] = src[r * ldb + c];
+        }
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int e = e0 + k * nt, r = e / NB, c = e - r * NB;
+            if (e < total) dst[(int64_t)r * ldd + c] = v[k];
+        }
+    }
+}
+
 template <typename T, int RPL>
 __global__ __launch_bounds__(256)
 void k_getrf_panel(T *A, int64_t lda, int M, int NB, int R, int kglob, int64_t *ipiv,
@@ -309,10 +351,7 @@ void k_getrf_panel(T *A, int64_t lda, int M, int NB, int R, int kglob, int64_t *
     const int rowbase = w * R;
     const int nrows = min(R, M - rowbase);
 
-    for (int e = tid; e < nrows * NB; e += blockDim.x) {
-        int r = e / NB, c = e - r * NB;
-        blk[r * LDB + c] = A[(int64_t)(rowbase + r) * lda + c];
-    }
+    stage_block(blk, LDB, A + (int64_t)rowbase * lda, lda, nrows, NB);
     if (tid == 0) s_abort = 0;
     __syncthreads();
 
@@ -624,10 +663,7 @@ void k_getrf_panel(T *A, int64_t lda, int M, int NB, int R, int kglob, int64_t *
     if (w == 0 && tid == 0)
         for (int k = 0; k < 8; ++k) atomicAdd(&g_panel_phase[k], (unsigned long long)ph_acc[k]);
 #endif
-    for (int e = tid; e < nrows * NB; e += blockDim.x) {
-        int r = e / NB, c = e - r * NB;
-        A[(int64_t)(rowbase + r) * lda + c] = blk[r * LDB + c];
-    }
+    unstage_block(A + (int64_t)rowbase * lda, lda, blk, LDB, nrows, NB);
     __syncthreads();
     if (w == 0 && tid < 64) panel_net_moves(NB, kglob, s_piv, swapmap);
     // L11 (rows 0..NB-1 of the panel, all owned by workgroup 0 since R >= NB)
@@ -657,10 +693,7 @@ void k_getrf_panel_local(T *A, int64_t lda, int M, int NB, int kglob, int64_t *i
     __shared__ int s_piv[NB_IN];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
-    for (int e = tid; e < M * NB; e += blockDim.x) {
-        int r = e / NB, c = e - r * NB;
-        blk[r * LDB + c] = A[(int64_t)r * lda + c];
-    }
+    stage_block(blk, LDB, A, lda, M, NB);
     __syncthreads();
 
     for (int j = 0; j < NB; ++j) {
@@ -718,10 +751,7 @@ void k_getrf_panel_local(T *A, int64_t lda, int M, int NB, int kglob, int64_t *i
                 blk[i * LDB + j] = blk[i * LDB + j] / piv;
         __syncthreads();
     }
-    for (int e = tid; e < M * NB; e += blockDim.x) {
-        int r = e / NB, c = e - r * NB;
-        A[(int64_t)r * lda + c] = blk[r * LDB + c];
-    }
+    unstage_block(A, lda, blk, LDB, M, NB);
     __syncthreads();
     if (tid < 64) panel_net_moves(NB, kglob, s_piv, swapmap);
     if (wave == 1) invert_unit_lower_wave<T>(blk, LDB, NB, linv);
