@@ -801,8 +801,22 @@ void k_apply_rowmoves(T *A, int64_t lda, int n, int c_lo, int c_hi,
             if (tid < cnt) perm[s_dst[tid]] = pv;
             continue;
         }
-        for (int e = tid >> 6; e < cnt; e += blockDim.x >> 6)
-            tile[e][cl] = valid ? A[s_src[e] * lda + col] : T(0);
+        // eight rows per wave in flight: loads from clamped positions (a
+        // lane past the last column reads column 0, never stored), issued
+        // before any is stored
+        for (int e0 = tid >> 6; e0 < cnt; e0 += 8 * (blockDim.x >> 6)) {
+            T rv[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int e = min(e0 + k * (int)(blockDim.x >> 6), cnt - 1);
+                rv[k] = A[s_src[e] * lda + (valid ? col : 0)];
+            }
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int e = e0 + k * (int)(blockDim.x >> 6);
+                if (e < cnt) tile[e][cl] = rv[k];
+            }
+        }
         __syncthreads();
         if (valid)
             for (int e = tid >> 6; e < cnt; e += blockDim.x >> 6)
