@@ -100,10 +100,20 @@ void k_qr_panel(int m, double *__restrict__ A, int64_t lda, int k0, int nb,
     __shared__ int s_abort;
     const int t = threadIdx.x, w = blockIdx.x, P = gridDim.x;
     const int r0 = k0 + w * R;
-    for (int i = t; i < R * QR_NB; i += 256) {
-        const int lr = i / QR_NB, j = i % QR_NB;
-        const double v = A[(int64_t)min(r0 + lr, m - 1) * lda + k0 + min(j, nb - 1)];
-        qx[lr * QRP_LS + j] = (r0 + lr < m && j < nb) ? v : 0.0;
+    // sixteen loads per thread (clamped, in-bounds positions) issued before
+    // any is stored: one memory round trip per sixteen elements
+    for (int i0 = t; i0 < R * QR_NB; i0 += 16 * 256) {
+        double v[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int i = i0 + 256 * k, lr = i / QR_NB, j = i % QR_NB;
+            v[k] = A[(int64_t)min(r0 + lr, m - 1) * lda + k0 + min(j, nb - 1)];
+        }
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int i = i0 + 256 * k, lr = i / QR_NB, j = i % QR_NB;
+            qx[lr * QRP_LS + j] = (r0 + lr < m && j < nb) ? v[k] : 0.0;
+        }
     }
     if (t == 0) s_abort = 0;
     if (t < QR_NB) s_av[t] = 0.0;
@@ -113,15 +123,20 @@ void k_qr_panel(int m, double *__restrict__ A, int64_t lda, int k0, int nb,
         const int gc = k0 + c;
         const uint32_t tag = (uint32_t)gc;
         const int buf = c & 1;
-        // 1. this workgroup's partial sums
+        // 1. this workgroup's partial sums: four interleaved accumulators
+        //    (rows q mod 4), added as (a0 + a1) + (a2 + a3), so the
+        //    dependent FP64 chain is GR / 4 fmas long instead of GR
         {
-            double acc = 0.0;
-            for (int q = 0; q < GR; ++q) {
-                const int lr = g * GR + q;
-                const double x = r0 + lr >= gc ? qx[lr * QRP_LS + c] : 0.0;
-                acc = fma(x, qx[lr * QRP_LS + jt], acc);
+            double acc[4] = {0.0, 0.0, 0.0, 0.0};
+            for (int q = 0; q < GR; q += 4) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int lr = g * GR + q + u;
+                    const double x = r0 + lr >= gc ? qx[lr * QRP_LS + c] : 0.0;
+                    acc[u] = fma(x, qx[lr * QRP_LS + jt], acc[u]);
+                }
             }
-            red[g][jt] = acc;
+            red[g][jt] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
         }
         __syncthreads();
         // 2. publish
