@@ -944,9 +944,24 @@ void k_u12_block(const T *__restrict__ Lblk, int64_t lda, const T *__restrict__ 
         }
     };
     load_stage(0);
-    for (int i = rq; i < NB_OUT; i += RQ) {
-        const T v = B[(int64_t)min(i, nbo - 1) * ldb + col];
-        X[i * XS + cl] = i < nbo ? v : T(0);
+    if constexpr (sizeof(T) == 8) {
+        // FP64: this thread's NB_OUT / RQ rows of the block, all loads in
+        // flight before the first store (measured: 1% off the FP64 LU; the
+        // same change slows the FP32 instance, which keeps the loop)
+        constexpr int NX = NB_OUT / RQ;
+        T xv[NX];
+#pragma unroll
+        for (int k = 0; k < NX; ++k) xv[k] = B[(int64_t)min(rq + RQ * k, nbo - 1) * ldb + col];
+#pragma unroll
+        for (int k = 0; k < NX; ++k) {
+            const int i = rq + RQ * k;
+            X[i * XS + cl] = i < nbo ? xv[k] : T(0);
+        }
+    } else {
+        for (int i = rq; i < NB_OUT; i += RQ) {
+            const T v = B[(int64_t)min(i, nbo - 1) * ldb + col];
+            X[i * XS + cl] = i < nbo ? v : T(0);
+        }
     }
     store_stage(0, Ls);
     for (int st = 0; st < nstage; ++st) {
